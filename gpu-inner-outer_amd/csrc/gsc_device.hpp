@@ -1,0 +1,236 @@
+// gsc_device.hpp — data layout and FP64 cell-box arithmetic shared by the
+// host (table building) and the gfx950 kernels.
+//
+// Every function here that turns FP64 coordinates into integer cells must give
+// the same bits on the host and on the device. Both sides are compiled with
+// -ffp-contract=off (no FMA contraction), keep the reference's expression
+// order (grid.hpp:72, 106, 211-281; model.hpp:34-45, 59-75, 107-111) and use
+// only IEEE-exact operations (+, -, *, correctly rounded /, floor, ceil).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "gsc_amd.h"
+
+namespace gsc_amd {
+
+#define GSC_HD __host__ __device__
+
+constexpr double kEps = 1e-9;  // grid.hpp:216
+
+// ---- pair record (4 bytes per (state, input) pair) ------------------------
+// The reference stores min_flat u32 + max_flat u32 + valid u8 per pair
+// (abstraction.hpp:32-38). Here the successor box is split: its θ and v ranges
+// depend only on (θ cell, ω input) and (v cell, a input) — the θ and v
+// trajectories of model.hpp:26-46 do not involve x or y — so they live in two
+// small tables (box2, box3). The per-pair word keeps the x and y ranges,
+// relative to the state's own cell:
+//   bits  0- 9  x offset  (box.min[0] - ix, two's complement, -512..511)
+//   bits 10-14  x extent  (box.max[0] - box.min[0], 0..31)
+//   bits 15-24  y offset
+//   bits 25-29  y extent
+//   bit  30     dead: the box holds no valid cell, so the pair can never be
+//               satisfied (the reference's `valid_count == 0` test,
+//               reachability.hpp:387-394 / kernels.cu:327-332, hoisted: the
+//               valid set never changes)
+//   bit  31     escape: bits 0-30 index an overflow entry holding the full
+//               (min_flat, max_flat) pair; used for boxes that do not fit the
+//               fields above (wrapped x/y ranges, large boxes, inverted ranges)
+// 0xFFFFFFFF marks an invalid pair (valid = 0 in the reference).
+constexpr uint32_t kRecInvalid = 0xFFFFFFFFu;
+constexpr uint32_t kRecEscape = 0x80000000u;
+constexpr uint32_t kRecDead = 0x40000000u;
+constexpr uint32_t kRecIndexMask = 0x7FFFFFFFu;
+constexpr int kOffBits = 10;
+constexpr int kOffMin = -(1 << (kOffBits - 1));
+constexpr int kOffMax = (1 << (kOffBits - 1)) - 1;
+constexpr uint32_t kExtMax = 31;
+// box2/box3 entry whose range is out of the state domain (pair invalid).
+constexpr uint32_t kBoxReject = 0xFFFFFFFFu;
+
+GSC_HD inline uint32_t rec_pack(int off0, uint32_t ext0, int off1, uint32_t ext1, bool dead) {
+    return (static_cast<uint32_t>(off0) & 0x3FFu) | (ext0 << 10) | ((static_cast<uint32_t>(off1) & 0x3FFu) << 15) |
+           (ext1 << 25) | (dead ? kRecDead : 0u);
+}
+GSC_HD inline int rec_off0(uint32_t r) { return static_cast<int32_t>(r << 22) >> 22; }
+GSC_HD inline uint32_t rec_ext0(uint32_t r) { return (r >> 10) & 31u; }
+GSC_HD inline int rec_off1(uint32_t r) { return static_cast<int32_t>(r << 7) >> 22; }
+GSC_HD inline uint32_t rec_ext1(uint32_t r) { return (r >> 25) & 31u; }
+
+// Overflow entry for escaped pairs.
+struct OverflowEntry {
+    uint32_t min_flat;
+    uint32_t max_flat;
+    uint32_t dead;
+    uint32_t reserved;
+};
+
+// ---- case parameters passed by value to every kernel -----------------------
+struct DevCase {
+    double lb[4], ub[4], eta[4];
+    double lb_minus_eps[4];  // lb[d] - kEps   (grid.hpp:223)
+    double ub_plus_eps[4];   // ub[d] + kEps
+    double wrap_span_minus_eps;  // (ub[w] - lb[w]) - kEps (grid.hpp:228-233)
+    uint32_t shape[4];
+    uint32_t stride[4];   // state strides, x fastest (grid.hpp:156-159)
+    uint32_t pstride[4];  // padded prefix strides (grid.hpp:283-299)
+    uint32_t S;           // states
+    uint32_t I;           // inputs
+    uint32_t n_a;         // input shape[0]: a (dv = u[0], model.hpp:39)
+    uint32_t n_w;         // input shape[1]: ω (dθ = u[1], model.hpp:38)
+    uint32_t K;           // integration substeps
+    int32_t wrap_dim;
+    double dt;            // sample_time / substeps (model.hpp:34)
+    int32_t constraint_type;
+    double hyp_a, hyp_b, hyp_c, ell_a, ell_b;
+    gsc_rect4 candidate, map;
+    int32_t band_enabled;
+    double band_c, band_w;
+    uint32_t n_obstacles, n_targets;
+};
+
+GSC_HD inline double cell_center(double lb, uint32_t idx, double eta) {
+    // grid.hpp:72 / :106
+    return lb + (static_cast<double>(idx) + 0.5) * eta;
+}
+
+// config.cpp:64-75, 111-127 (the constraint reads x[0] and x[1] only).
+GSC_HD inline bool constraint_ok(int type, double ha, double hb, double hc, double ea, double eb, double x0,
+                                 double x1) {
+    if (type == GSC_CONSTRAINT_HYPERBOLIC) {
+        const double x1_sq = x0 * x0;
+        const double x2_sq = x1 * x1;
+        return (x1_sq - x2_sq <= ha) && (hb * x2_sq - x1_sq <= hc);
+    }
+    if (type == GSC_CONSTRAINT_ELLIPTIC) {
+        const double t1 = (x0 / ea) * (x0 / ea);
+        const double t2 = (x1 / eb) * (x1 / eb);
+        return t1 + t2 <= 1.0;
+    }
+    return true;
+}
+
+GSC_HD inline bool point_in_rect(const gsc_rect4& r, const double x[4]) {
+    // config.cpp:79-86
+    for (int d = 0; d < 4; ++d)
+        if (x[d] < r.lb[d] || x[d] > r.ub[d]) return false;
+    return true;
+}
+
+// Unicycle4DModel::wrap_angle, model.hpp:14-24 (literal π, not the grid bounds).
+GSC_HD inline double wrap_angle(double x) {
+    constexpr double kPi = 3.14159265358979323846;
+    constexpr double kTwoPi = 2.0 * kPi;
+    while (x < -kPi) x += kTwoPi;
+    while (x >= kPi) x -= kTwoPi;
+    return x;
+}
+
+// Grid4D::wrap_value, grid.hpp:76-87.
+GSC_HD inline double wrap_value(double low, double high, double x) {
+    const double span = high - low;
+    while (x < low) x += span;
+    while (x >= high) x -= span;
+    return x;
+}
+
+// One dimension of continuous_box_to_index_box (grid.hpp:211-281).
+// Returns false when a non-wrap dimension leaves [lb - eps, ub + eps].
+GSC_HD inline bool dim_to_cells(const DevCase& c, int d, double mn, double mx, uint32_t* lo, uint32_t* hi) {
+    double a = mn, b = mx;
+    if (d == c.wrap_dim) {
+        const double raw_width = mx - mn;
+        if (raw_width >= c.wrap_span_minus_eps) {
+            *lo = 0;
+            *hi = c.shape[d] - 1;
+            return true;
+        }
+        a = wrap_value(c.lb[d], c.ub[d], mn);
+        b = wrap_value(c.lb[d], c.ub[d], mx);
+    } else {
+        if (mn < c.lb_minus_eps[d] || mx > c.ub_plus_eps[d]) return false;
+    }
+    int64_t lower = static_cast<int64_t>(floor((a - c.lb[d]) / c.eta[d]));
+    int64_t upper = static_cast<int64_t>(ceil((b - c.lb[d]) / c.eta[d])) - 1;
+    const int64_t n = static_cast<int64_t>(c.shape[d]);
+    if (lower < 0) lower = 0;
+    if (upper < 0) upper = 0;
+    if (lower >= n) lower = n - 1;
+    if (upper >= n) upper = n - 1;
+    *lo = static_cast<uint32_t>(lower);
+    *hi = static_cast<uint32_t>(upper);
+    return true;
+}
+
+// 16-corner inclusion–exclusion on a padded u32 prefix table (prefix_sum.hpp:65-85)
+// for a box with lo <= hi in every dimension. Exact: the true count lies in [0, S].
+GSC_HD inline uint32_t box_count_u32(const uint32_t* __restrict__ P, const uint32_t ps[4], const uint32_t lo[4],
+                                     const uint32_t hi[4]) {
+    const uint32_t x0 = lo[0], x1 = hi[0] + 1;
+    const uint32_t y0 = lo[1] * ps[1], y1 = (hi[1] + 1) * ps[1];
+    const uint32_t z0 = lo[2] * ps[2], z1 = (hi[2] + 1) * ps[2];
+    const uint32_t w0 = lo[3] * ps[3], w1 = (hi[3] + 1) * ps[3];
+    const uint32_t a11 = w1 + z1, a10 = w1 + z0, a01 = w0 + z1, a00 = w0 + z0;
+    uint32_t s = 0;
+    s += P[x1 + y1 + a11];
+    s -= P[x0 + y1 + a11];
+    s -= P[x1 + y0 + a11];
+    s += P[x0 + y0 + a11];
+    s -= P[x1 + y1 + a10];
+    s += P[x0 + y1 + a10];
+    s += P[x1 + y0 + a10];
+    s -= P[x0 + y0 + a10];
+    s -= P[x1 + y1 + a01];
+    s += P[x0 + y1 + a01];
+    s += P[x1 + y0 + a01];
+    s -= P[x0 + y0 + a01];
+    s += P[x1 + y1 + a00];
+    s -= P[x0 + y1 + a00];
+    s -= P[x1 + y0 + a00];
+    s += P[x0 + y0 + a00];
+    return s;
+}
+
+// query_box_count (prefix_sum.hpp:87-105): a box whose wrap-dimension range
+// wraps (min > max) is split in two.
+GSC_HD inline uint32_t box_count_wrap_u32(const uint32_t* __restrict__ P, const uint32_t ps[4], int wd,
+                                          const uint32_t shape_wd, const uint32_t lo[4], const uint32_t hi[4]) {
+    if (lo[wd] <= hi[wd]) return box_count_u32(P, ps, lo, hi);
+    uint32_t hl[4] = {hi[0], hi[1], hi[2], hi[3]};
+    uint32_t lr[4] = {lo[0], lo[1], lo[2], lo[3]};
+    hl[wd] = shape_wd - 1;
+    lr[wd] = 0;
+    return box_count_u32(P, ps, lo, hl) + box_count_u32(P, ps, lr, hi);
+}
+
+// Same query with signed 64-bit accumulation, exact for any (even inverted)
+// box: used for escaped pairs, where the reference's u64 modular sums of
+// inverted ranges must be matched as integers.
+GSC_HD inline int64_t box_count_i64_single(const uint32_t* __restrict__ P, const uint32_t ps[4], const uint32_t lo[4],
+                                           const uint32_t hi[4]) {
+    int64_t s = 0;
+    for (int m = 0; m < 16; ++m) {
+        uint64_t off = 0;
+        for (int d = 0; d < 4; ++d) {
+            const uint64_t v = (m >> d) & 1 ? lo[d] : static_cast<uint64_t>(hi[d]) + 1;
+            off += v * ps[d];
+        }
+        const int64_t val = static_cast<int64_t>(P[off]);
+        s += (__builtin_popcount(static_cast<unsigned>(m)) & 1) ? -val : val;
+    }
+    return s;
+}
+
+GSC_HD inline int64_t box_count_i64(const uint32_t* __restrict__ P, const uint32_t ps[4], int wd, uint32_t shape_wd,
+                                    const uint32_t lo[4], const uint32_t hi[4]) {
+    if (lo[wd] <= hi[wd]) return box_count_i64_single(P, ps, lo, hi);
+    uint32_t hl[4] = {hi[0], hi[1], hi[2], hi[3]};
+    uint32_t lr[4] = {lo[0], lo[1], lo[2], lo[3]};
+    hl[wd] = shape_wd - 1;
+    lr[wd] = 0;
+    return box_count_i64_single(P, ps, lo, hl) + box_count_i64_single(P, ps, lr, hi);
+}
+
+}  // namespace gsc_amd

@@ -1,0 +1,89 @@
+// kernels.hpp — kernel entry points and the device-side control block.
+#pragma once
+
+#include "gsc_device.hpp"
+
+namespace gsc_amd {
+
+// Per-iteration counters live in a ring so the fixpoint loop never clears
+// memory with hipMemset between iterations: finish_kernel consumes slot
+// iter % kRing and zeroes it for reuse.
+constexpr uint32_t kRing = 64;
+constexpr uint32_t kDoneConverged = 1;      // "all candidate states obtained a controller"
+constexpr uint32_t kDoneStalled = 2;        // "iteration stalled before candidate closure"
+constexpr uint32_t kDoneMaxIterations = 3;  // "maximum iterations reached"
+
+struct IterCounters {
+    unsigned long long new_reachable;
+    unsigned long long new_certified;
+    unsigned long long pairs_tested;
+    unsigned long long pad;
+};
+
+struct IterCtl {
+    uint32_t done;        // 0 while iterating, else kDone*
+    uint32_t iterations;  // last finished iteration
+    unsigned long long candidate_count;
+    unsigned long long total_reachable;
+    unsigned long long total_certified;
+    IterCounters ring[kRing];
+};
+
+struct IterRecord {
+    uint32_t iteration;
+    uint32_t pad;
+    unsigned long long newly_reachable;
+    unsigned long long newly_certified;
+    unsigned long long total_reachable;
+    unsigned long long total_certified;
+    unsigned long long pairs_tested;
+};
+
+struct AbstractionArgs {
+    DevCase c;
+    const double2* cs;           // [n_θ][n_ω][K] (cos θ_k, sin θ_k)
+    const double* vt;            // [n_v][n_a][K] v_k
+    const uint2* box2;           // [n_θ][n_ω] θ cell range (or kBoxReject)
+    const uint2* box3;           // [n_v][n_a] v cell range (or kBoxReject)
+    const double* rad;           // [n_v][2] x / y box radius
+    const uint32_t* fail2d;      // (n0+1)x(n1+1) prefix of constraint failures, or null
+    const uint32_t* prefix_valid;
+    uint32_t* rec;
+    OverflowEntry* overflow;
+    uint32_t* overflow_count;
+    uint32_t overflow_capacity;
+    uint32_t in_chunk;
+};
+
+struct SatisfyArgs {
+    DevCase c;
+    const uint32_t* prefix_unreached;
+    const uint32_t* rec;
+    const OverflowEntry* overflow;
+    const uint2* box2;
+    const uint2* box3;
+    const uint8_t* valid;
+    const uint8_t* cand;
+    uint8_t* reach;
+    uint32_t* ctrl;
+    uint32_t* rstep;
+    IterCtl* ctl;
+    uint32_t iter;
+    uint32_t chunk_states;
+    uint32_t sweep_all;
+};
+
+__global__ void masks_kernel(DevCase c, const gsc_rect4* obstacles, const gsc_rect4* targets, uint8_t* valid,
+                             uint8_t* cand, unsigned long long* counts);
+__global__ void seed_kernel(uint32_t S, const uint8_t* cand, uint8_t* reach, uint32_t* ctrl, uint32_t* rstep);
+__global__ void reset_ctl_kernel(IterCtl* ctl, unsigned long long candidate_count);
+__global__ void prefix_rows_kernel(DevCase c, const uint8_t* a, const uint8_t* b, uint32_t* P, const IterCtl* ctl);
+__global__ void prefix_axis_kernel(DevCase c, int axis, uint32_t* P, const IterCtl* ctl);
+__global__ void abstraction_kernel(AbstractionArgs A);
+__global__ void export_kernel(DevCase c, const uint32_t* rec, const uint2* box2, const uint2* box3,
+                              const OverflowEntry* overflow, uint64_t pair_begin, uint64_t pair_end,
+                              uint32_t* out_min, uint32_t* out_max, uint8_t* out_valid);
+__global__ void satisfy_kernel(SatisfyArgs A);
+__global__ void finish_kernel(IterCtl* ctl, IterRecord* records, uint32_t iter, uint32_t max_iterations);
+
+}  // namespace gsc_amd

@@ -1,0 +1,814 @@
+// solver.hip — host orchestration of the gfx950 backend and the C ABI.
+//
+// A gsc_solver owns one case on one device: its tables, masks, pair records,
+// prefix tables and iteration state, all resident in HBM. The fixpoint loop
+// (kernels.cu:654-767 / reachability.hpp:362-452) is launched in batches of
+// iterations with no host synchronisation inside a batch: every kernel reads a
+// device-side `done` flag and returns at once after the stop test fired, so a
+// batch past the last iteration costs a few empty launches and the stop
+// iteration and message are exactly the reference's.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace gsc_amd {
+namespace {
+
+struct DeviceError : std::runtime_error {
+    hipError_t code;
+    DeviceError(hipError_t c, const std::string& what) : std::runtime_error(what), code(c) {}
+};
+struct ConfigError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define GSC_HIP(expr)                                                                             \
+    do {                                                                                          \
+        hipError_t e__ = (expr);                                                                  \
+        if (e__ != hipSuccess) throw DeviceError(e__, hipGetErrorString(e__));                    \
+    } while (0)
+
+constexpr uint32_t kBatch = 8;          // iterations launched per host round trip
+constexpr uint32_t kEventsPerIter = 4;  // start, after prefix, after satisfy, after finish
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void put(char* dst, size_t len, const std::string& s) {
+    if (dst && len) std::snprintf(dst, len, "%s", s.c_str());
+}
+
+uint32_t blocks_for(uint64_t items, uint32_t per_block) {
+    const uint64_t b = (items + per_block - 1) / per_block;
+    if (b > 0x7FFFFFFFull) throw ConfigError("kernel launch grid exceeds the device limit");
+    return static_cast<uint32_t>(b);
+}
+
+const char* done_message(uint32_t done) {
+    switch (done) {
+        case kDoneConverged: return "all candidate states obtained a controller";
+        case kDoneStalled: return "iteration stalled before candidate closure";
+        default: return "maximum iterations reached";
+    }
+}
+
+// build_state_grid / build_input_grid / build_prefix_layout (grid.hpp:133-198, 283-299),
+// with the reference's error texts.
+gsc_case_shape make_shape(const gsc_case_desc& c) {
+    gsc_case_shape sh;
+    std::memset(&sh, 0, sizeof(sh));
+    uint64_t total = 1;
+    for (int d = 0; d < 4; ++d) {
+        if (!(c.state_eta[d] > 0.0)) throw ConfigError("state eta must be positive");
+        const double width = c.state_ub[d] - c.state_lb[d];
+        if (!(width > 0.0)) throw ConfigError("state bounds must satisfy ub > lb");
+        sh.state_shape[d] = static_cast<uint32_t>(std::ceil(width / c.state_eta[d]));
+    }
+    sh.state_strides[0] = 1;
+    for (int d = 1; d < 4; ++d) sh.state_strides[d] = sh.state_strides[d - 1] * sh.state_shape[d - 1];
+    for (int d = 0; d < 4; ++d) total *= sh.state_shape[d];
+    sh.state_count = total;
+    if (total > 0xFFFFFFFFull) throw ConfigError("state grid exceeds the selected state index type");
+    for (int d = 0; d < 2; ++d) {
+        if (!(c.input_eta[d] > 0.0)) throw ConfigError("input eta must be positive");
+        const double width = c.input_ub[d] - c.input_lb[d];
+        if (!(width > 0.0)) throw ConfigError("input bounds must satisfy ub > lb");
+        sh.input_shape[d] = static_cast<uint32_t>(std::ceil(width / c.input_eta[d]));
+    }
+    sh.input_strides[0] = 1;
+    sh.input_strides[1] = sh.input_shape[0];
+    sh.input_count = static_cast<uint64_t>(sh.input_shape[0]) * sh.input_shape[1];
+    if (sh.input_count > 0xFFFFFFFFull) throw ConfigError("input grid exceeds uint32_t");
+    sh.pair_count = sh.state_count * sh.input_count;
+    uint64_t pt = 1;
+    for (int d = 0; d < 4; ++d) {
+        sh.prefix_shape[d] = sh.state_shape[d] + 1;
+        pt *= sh.prefix_shape[d];
+    }
+    sh.prefix_count = pt;
+    return sh;
+}
+
+// estimate_memory_budget, abstraction.hpp:211-243 (u32 state index, u64 prefix counts).
+gsc_memory_budget make_budget(const gsc_case_shape& sh) {
+    gsc_memory_budget b;
+    std::memset(&b, 0, sizeof(b));
+    const uint64_t P = sh.pair_count, S = sh.state_count;
+    b.pair_min_bytes = P * 4;
+    b.pair_max_bytes = P * 4;
+    b.pair_valid_bytes = P;
+    b.pair_satisfied_bytes = P;
+    b.reachable_prev_bytes = S;
+    b.reachable_next_bytes = S;
+    b.valid_mask_bytes = S;
+    b.candidate_mask_bytes = S;
+    b.controller_bytes = S * 4;
+    b.reach_step_bytes = S * 4;
+    b.prefix_reachable_bytes = sh.prefix_count * 8;
+    b.prefix_valid_bytes = sh.prefix_count * 8;
+    b.total_bytes = b.pair_min_bytes + b.pair_max_bytes + b.pair_valid_bytes + b.pair_satisfied_bytes +
+                    b.reachable_prev_bytes + b.reachable_next_bytes + b.valid_mask_bytes + b.candidate_mask_bytes +
+                    b.controller_bytes + b.reach_step_bytes + b.prefix_reachable_bytes + b.prefix_valid_bytes;
+    return b;
+}
+
+}  // namespace
+}  // namespace gsc_amd
+
+using namespace gsc_amd;
+
+struct gsc_solver {
+    gsc_solver_options opts{};
+    std::string name;
+    std::vector<gsc_rect4> obstacles, targets;
+    gsc_case_desc cfg{};
+    gsc_case_shape shape{};
+    DevCase dc{};
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocations;
+    uint64_t bytes_allocated = 0;
+
+    uint8_t *valid = nullptr, *cand = nullptr, *reach = nullptr;
+    uint32_t *ctrl = nullptr, *rstep = nullptr;
+    uint32_t *prefix_valid = nullptr, *prefix_unreached = nullptr;
+    uint32_t* rec = nullptr;
+    OverflowEntry* overflow = nullptr;
+    uint32_t overflow_capacity = 0;
+    uint32_t* overflow_count = nullptr;
+    double2* cs = nullptr;
+    double* vt = nullptr;
+    uint2 *box2 = nullptr, *box3 = nullptr;
+    double* rad = nullptr;
+    uint32_t* fail2d = nullptr;
+    gsc_rect4 *d_obstacles = nullptr, *d_targets = nullptr;
+    unsigned long long* mask_counts = nullptr;
+    IterCtl* ctl = nullptr;
+    IterRecord* records = nullptr;
+    IterCtl* host_ctl = nullptr;        // pinned
+    IterRecord* host_records = nullptr; // pinned
+    std::vector<hipEvent_t> events;
+
+    uint64_t candidate_count = 0;
+    bool abstraction_ready = false;
+    bool solved = false;
+    uint32_t in_chunk = 0;
+    uint32_t sat_chunk = 0;
+    size_t sat_shmem = 0;
+
+    double abstraction_kernel_ms = 0.0;
+    double abstraction_wall_ms = 0.0;
+    double setup_h2d_ms = 0.0;
+    gsc_solve_summary summary{};
+    std::vector<gsc_iteration_stats> iter_stats;
+    gsc_kernel_timings timings{};
+    std::string last_error;
+
+    template <typename T>
+    T* alloc(uint64_t count) {
+        void* p = nullptr;
+        const uint64_t bytes = std::max<uint64_t>(count, 1) * sizeof(T);
+        GSC_HIP(hipMalloc(&p, bytes));
+        allocations.push_back(p);
+        bytes_allocated += bytes;
+        return static_cast<T*>(p);
+    }
+
+    ~gsc_solver() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void* p : allocations) (void)hipFree(p);
+        if (host_ctl) (void)hipHostFree(host_ctl);
+        if (host_records) (void)hipHostFree(host_records);
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void build_tables(std::vector<double2>& h_cs, std::vector<double>& h_vt, std::vector<uint2>& h_box2,
+                      std::vector<uint2>& h_box3, std::vector<double>& h_rad, std::vector<uint32_t>& h_fail);
+    void setup();
+    void build_prefix(const uint8_t* a, const uint8_t* b, uint32_t* P, const IterCtl* ctl_or_null);
+    void build_abstraction();
+    void solve();
+    void download(const gsc_result_buffers* out, gsc_run_report* report);
+};
+
+// Host tables. The θ trajectory depends on (θ cell, ω) only and the v trajectory
+// on (v cell, a) only (model.hpp:35-45), so cos θ_k, sin θ_k and v_k are filled
+// here once per combination with the host libm, in the reference's order:
+//   θ_{k+1} = wrap_angle(θ_k + dt*ω),  v_{k+1} = v_k + dt*a.
+// The θ and v cell ranges of the successor box follow from θ_K, v_K and the
+// constant radii h2 + T*dw2, h3 + T*dw3 (model.hpp:74-75, 84-89, 107-111).
+void gsc_solver::build_tables(std::vector<double2>& h_cs, std::vector<double>& h_vt, std::vector<uint2>& h_box2,
+                              std::vector<uint2>& h_box3, std::vector<double>& h_rad, std::vector<uint32_t>& h_fail) {
+    const uint32_t n2 = dc.shape[2], n3 = dc.shape[3], na = dc.n_a, nw = dc.n_w, K = dc.K;
+    const double T = cfg.sample_time;
+    const double dt = dc.dt;
+    const double h[4] = {0.5 * cfg.state_eta[0], 0.5 * cfg.state_eta[1], 0.5 * cfg.state_eta[2],
+                         0.5 * cfg.state_eta[3]};
+    h_cs.assign(static_cast<size_t>(n2) * nw * K, double2{0.0, 0.0});
+    h_box2.assign(static_cast<size_t>(n2) * nw, uint2{0, 0});
+    const double r2 = h[2] + T * cfg.disturbance_half_width[2];
+    for (uint32_t iz = 0; iz < n2; ++iz) {
+        const double th0 = cell_center(cfg.state_lb[2], iz, cfg.state_eta[2]);
+        for (uint32_t iw = 0; iw < nw; ++iw) {
+            const double omega = cell_center(cfg.input_lb[1], iw, cfg.input_eta[1]);
+            double th = th0;
+            double2* row = &h_cs[(static_cast<size_t>(iz) * nw + iw) * K];
+            for (uint32_t k = 0; k < K; ++k) {
+                row[k].x = std::cos(th);
+                row[k].y = std::sin(th);
+                th = wrap_angle(th + dt * omega);
+            }
+            uint32_t lo = 0, hi = 0;
+            if (dim_to_cells(dc, 2, th - r2, th + r2, &lo, &hi))
+                h_box2[static_cast<size_t>(iz) * nw + iw] = uint2{lo, hi};
+            else
+                h_box2[static_cast<size_t>(iz) * nw + iw] = uint2{kBoxReject, kBoxReject};
+        }
+    }
+    h_vt.assign(static_cast<size_t>(n3) * na * K, 0.0);
+    h_box3.assign(static_cast<size_t>(n3) * na, uint2{0, 0});
+    h_rad.assign(static_cast<size_t>(n3) * 2, 0.0);
+    const double r3 = h[3] + T * cfg.disturbance_half_width[3];
+    for (uint32_t iv = 0; iv < n3; ++iv) {
+        const double v0 = cell_center(cfg.state_lb[3], iv, cfg.state_eta[3]);
+        const double abs_v = std::fabs(v0);
+        const double jfx0 = h[0] + T * abs_v * h[2] + T * h[3];
+        const double jfx1 = h[1] + T * abs_v * h[2] + T * h[3];
+        h_rad[2 * iv] = jfx0 + T * cfg.disturbance_half_width[0];
+        h_rad[2 * iv + 1] = jfx1 + T * cfg.disturbance_half_width[1];
+        for (uint32_t ia = 0; ia < na; ++ia) {
+            const double acc = cell_center(cfg.input_lb[0], ia, cfg.input_eta[0]);
+            double v = v0;
+            double* row = &h_vt[(static_cast<size_t>(iv) * na + ia) * K];
+            for (uint32_t k = 0; k < K; ++k) {
+                row[k] = v;
+                v += dt * acc;
+            }
+            uint32_t lo = 0, hi = 0;
+            if (dim_to_cells(dc, 3, v - r3, v + r3, &lo, &hi))
+                h_box3[static_cast<size_t>(iv) * na + ia] = uint2{lo, hi};
+            else
+                h_box3[static_cast<size_t>(iv) * na + ia] = uint2{kBoxReject, kBoxReject};
+        }
+    }
+    h_fail.clear();
+    if (cfg.constraint_type != GSC_CONSTRAINT_NONE) {
+        const uint32_t n0 = dc.shape[0], n1 = dc.shape[1];
+        const uint32_t pitch = n0 + 1;
+        h_fail.assign(static_cast<size_t>(n0 + 1) * (n1 + 1), 0u);
+        for (uint32_t iy = 0; iy < n1; ++iy) {
+            const double yc = cell_center(cfg.state_lb[1], iy, cfg.state_eta[1]);
+            uint32_t row = 0;
+            for (uint32_t ix = 0; ix < n0; ++ix) {
+                const double xc = cell_center(cfg.state_lb[0], ix, cfg.state_eta[0]);
+                const bool ok = constraint_ok(cfg.constraint_type, cfg.hyperbolic_a, cfg.hyperbolic_b,
+                                              cfg.hyperbolic_c, cfg.elliptic_a, cfg.elliptic_b, xc, yc);
+                row += ok ? 0u : 1u;
+                h_fail[static_cast<size_t>(iy + 1) * pitch + ix + 1] = h_fail[static_cast<size_t>(iy) * pitch + ix + 1] + row;
+            }
+        }
+    }
+}
+
+void gsc_solver::setup() {
+    if (cfg.wrap_dim < 0 || cfg.wrap_dim > 3) throw ConfigError("wrap_dim must be 0, 1, 2 or 3");
+    shape = make_shape(cfg);
+    if (shape.prefix_count > 0xFFFFFFFFull) throw ConfigError("state grid too large for the 32-bit prefix layout");
+    if (shape.input_count > (1u << 19)) throw ConfigError("input grid larger than 524288 inputs is not supported");
+    if (shape.state_count * 4 >= 0xFFFFFFFFull) throw ConfigError("state grid too large for one launch");
+
+    dc = DevCase{};
+    for (int d = 0; d < 4; ++d) {
+        dc.lb[d] = cfg.state_lb[d];
+        dc.ub[d] = cfg.state_ub[d];
+        dc.eta[d] = cfg.state_eta[d];
+        dc.lb_minus_eps[d] = cfg.state_lb[d] - kEps;
+        dc.ub_plus_eps[d] = cfg.state_ub[d] + kEps;
+        dc.shape[d] = shape.state_shape[d];
+        dc.stride[d] = static_cast<uint32_t>(shape.state_strides[d]);
+    }
+    dc.pstride[0] = 1;
+    for (int d = 1; d < 4; ++d) dc.pstride[d] = dc.pstride[d - 1] * shape.prefix_shape[d - 1];
+    dc.wrap_dim = cfg.wrap_dim;
+    dc.wrap_span_minus_eps = (cfg.state_ub[cfg.wrap_dim] - cfg.state_lb[cfg.wrap_dim]) - kEps;
+    dc.S = static_cast<uint32_t>(shape.state_count);
+    dc.I = static_cast<uint32_t>(shape.input_count);
+    dc.n_a = shape.input_shape[0];
+    dc.n_w = shape.input_shape[1];
+    dc.K = cfg.integration_substeps;
+    dc.dt = cfg.sample_time / static_cast<double>(cfg.integration_substeps);
+    dc.constraint_type = cfg.constraint_type;
+    dc.hyp_a = cfg.hyperbolic_a;
+    dc.hyp_b = cfg.hyperbolic_b;
+    dc.hyp_c = cfg.hyperbolic_c;
+    dc.ell_a = cfg.elliptic_a;
+    dc.ell_b = cfg.elliptic_b;
+    dc.candidate = cfg.candidate;
+    dc.map = cfg.map;
+    dc.band_enabled = cfg.target_band_enabled;
+    dc.band_c = cfg.target_band_c;
+    dc.band_w = cfg.target_band_w;
+    dc.n_obstacles = static_cast<uint32_t>(obstacles.size());
+    dc.n_targets = static_cast<uint32_t>(targets.size());
+
+    GSC_HIP(hipSetDevice(opts.device));
+    GSC_HIP(hipStreamCreate(&stream));
+    const uint64_t S = shape.state_count, P = shape.pair_count, PT = shape.prefix_count;
+    valid = alloc<uint8_t>(S);
+    cand = alloc<uint8_t>(S);
+    reach = alloc<uint8_t>(S);
+    ctrl = alloc<uint32_t>(S);
+    rstep = alloc<uint32_t>(S);
+    prefix_valid = alloc<uint32_t>(PT);
+    prefix_unreached = alloc<uint32_t>(PT);
+    rec = alloc<uint32_t>(P);
+    overflow_capacity = 1u << 16;
+    overflow = alloc<OverflowEntry>(overflow_capacity);
+    overflow_count = alloc<uint32_t>(1);
+    mask_counts = alloc<unsigned long long>(2);
+    ctl = alloc<IterCtl>(1);
+    records = alloc<IterRecord>(kRing);
+    GSC_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_ctl), sizeof(IterCtl), hipHostMallocDefault));
+    GSC_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_records), sizeof(IterRecord) * kRing, hipHostMallocDefault));
+    std::memset(host_ctl, 0, sizeof(IterCtl));
+    std::memset(host_records, 0, sizeof(IterRecord) * kRing);
+    events.resize(kBatch * kEventsPerIter + 2);
+    for (auto& e : events) GSC_HIP(hipEventCreate(&e));
+
+    std::vector<double2> h_cs;
+    std::vector<double> h_vt, h_rad;
+    std::vector<uint2> h_box2, h_box3;
+    std::vector<uint32_t> h_fail;
+    build_tables(h_cs, h_vt, h_box2, h_box3, h_rad, h_fail);
+    cs = alloc<double2>(h_cs.size());
+    vt = alloc<double>(h_vt.size());
+    box2 = alloc<uint2>(h_box2.size());
+    box3 = alloc<uint2>(h_box3.size());
+    rad = alloc<double>(h_rad.size());
+    if (!h_fail.empty()) fail2d = alloc<uint32_t>(h_fail.size());
+    d_obstacles = alloc<gsc_rect4>(obstacles.size());
+    d_targets = alloc<gsc_rect4>(targets.size());
+
+    const auto t0 = std::chrono::steady_clock::now();
+    GSC_HIP(hipMemcpyAsync(cs, h_cs.data(), h_cs.size() * sizeof(double2), hipMemcpyHostToDevice, stream));
+    GSC_HIP(hipMemcpyAsync(vt, h_vt.data(), h_vt.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    GSC_HIP(hipMemcpyAsync(box2, h_box2.data(), h_box2.size() * sizeof(uint2), hipMemcpyHostToDevice, stream));
+    GSC_HIP(hipMemcpyAsync(box3, h_box3.data(), h_box3.size() * sizeof(uint2), hipMemcpyHostToDevice, stream));
+    GSC_HIP(hipMemcpyAsync(rad, h_rad.data(), h_rad.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    if (fail2d)
+        GSC_HIP(hipMemcpyAsync(fail2d, h_fail.data(), h_fail.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    if (!obstacles.empty())
+        GSC_HIP(hipMemcpyAsync(d_obstacles, obstacles.data(), obstacles.size() * sizeof(gsc_rect4),
+                               hipMemcpyHostToDevice, stream));
+    if (!targets.empty())
+        GSC_HIP(hipMemcpyAsync(d_targets, targets.data(), targets.size() * sizeof(gsc_rect4), hipMemcpyHostToDevice,
+                               stream));
+    // Padding rows of both prefix tables stay zero for the life of the solver.
+    GSC_HIP(hipMemsetAsync(prefix_valid, 0, PT * sizeof(uint32_t), stream));
+    GSC_HIP(hipMemsetAsync(prefix_unreached, 0, PT * sizeof(uint32_t), stream));
+    GSC_HIP(hipMemsetAsync(mask_counts, 0, 2 * sizeof(unsigned long long), stream));
+
+    // Masks on the device (the reference builds them in a serial host loop even in
+    // GPU mode, abstraction.hpp:257-265).
+    hipLaunchKernelGGL(masks_kernel, dim3(blocks_for(S, 256)), dim3(256), 0, stream, dc, d_obstacles, d_targets, valid,
+                       cand, mask_counts);
+    GSC_HIP(hipGetLastError());
+    unsigned long long counts[2] = {0, 0};
+    GSC_HIP(hipMemcpyAsync(counts, mask_counts, sizeof(counts), hipMemcpyDeviceToHost, stream));
+    GSC_HIP(hipStreamSynchronize(stream));
+    setup_h2d_ms = ms_since(t0);
+    if (counts[1] != 0) throw ConfigError("candidate set contains invalid states");
+    candidate_count = counts[0];
+
+    build_prefix(valid, nullptr, prefix_valid, nullptr);
+
+    in_chunk = std::min<uint32_t>(dc.I, 128u);
+    const uint32_t max_bits = 65536u * 8u;
+    sat_chunk = std::min<uint32_t>(256u, max_bits / dc.I);
+    if (sat_chunk == 0) throw ConfigError("input grid too large");
+    sat_shmem = ((static_cast<size_t>(sat_chunk) * dc.I + 63) / 64) * sizeof(uint64_t);
+}
+
+void gsc_solver::build_prefix(const uint8_t* a, const uint8_t* b, uint32_t* P, const IterCtl* c) {
+    const uint64_t lines0 = static_cast<uint64_t>(dc.shape[1]) * dc.shape[2] * dc.shape[3];
+    hipLaunchKernelGGL(prefix_rows_kernel, dim3(blocks_for(lines0, 4)), dim3(256), 0, stream, dc, a, b, P, c);
+    GSC_HIP(hipGetLastError());
+    for (int axis = 1; axis <= 3; ++axis) {
+        const int o1 = axis == 1 ? 2 : 1;
+        const int o2 = axis == 3 ? 2 : 3;
+        const uint64_t lines = static_cast<uint64_t>(dc.shape[0]) * dc.shape[o1] * dc.shape[o2];
+        hipLaunchKernelGGL(prefix_axis_kernel, dim3(blocks_for(lines, 256)), dim3(256), 0, stream, dc, axis, P, c);
+        GSC_HIP(hipGetLastError());
+    }
+}
+
+void gsc_solver::build_abstraction() {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        GSC_HIP(hipMemsetAsync(overflow_count, 0, sizeof(uint32_t), stream));
+        AbstractionArgs A{};
+        A.c = dc;
+        A.cs = cs;
+        A.vt = vt;
+        A.box2 = box2;
+        A.box3 = box3;
+        A.rad = rad;
+        A.fail2d = fail2d;
+        A.prefix_valid = prefix_valid;
+        A.rec = rec;
+        A.overflow = overflow;
+        A.overflow_count = overflow_count;
+        A.overflow_capacity = overflow_capacity;
+        A.in_chunk = in_chunk;
+        const size_t shmem = static_cast<size_t>(64) * in_chunk * sizeof(uint32_t);
+        GSC_HIP(hipEventRecord(events[0], stream));
+        hipLaunchKernelGGL(abstraction_kernel, dim3(blocks_for(dc.S, 64)), dim3(256), shmem, stream, A);
+        GSC_HIP(hipGetLastError());
+        GSC_HIP(hipEventRecord(events[1], stream));
+        uint32_t used = 0;
+        GSC_HIP(hipMemcpyAsync(&used, overflow_count, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        GSC_HIP(hipStreamSynchronize(stream));
+        float ms = 0.f;
+        GSC_HIP(hipEventElapsedTime(&ms, events[0], events[1]));
+        abstraction_kernel_ms = ms;
+        if (used <= overflow_capacity) break;
+        if (used >= kRecIndexMask) throw ConfigError("too many irregular successor boxes");
+        overflow_capacity = used;
+        overflow = alloc<OverflowEntry>(overflow_capacity);
+    }
+    abstraction_wall_ms = ms_since(t0);
+    abstraction_ready = true;
+}
+
+void gsc_solver::solve() {
+    if (!abstraction_ready) build_abstraction();
+    const uint32_t S = dc.S;
+    hipLaunchKernelGGL(seed_kernel, dim3(blocks_for(S, 256)), dim3(256), 0, stream, S, cand, reach, ctrl, rstep);
+    GSC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(reset_ctl_kernel, dim3(1), dim3(64), 0, stream, ctl, static_cast<unsigned long long>(candidate_count));
+    GSC_HIP(hipGetLastError());
+    GSC_HIP(hipStreamSynchronize(stream));
+
+    iter_stats.clear();
+    std::memset(&summary, 0, sizeof(summary));
+    double prefix_ms = 0.0, sat_ms = 0.0, red_ms = 0.0;
+    const bool timed = opts.time_kernels != 0;
+    SatisfyArgs A{};
+    A.c = dc;
+    A.prefix_unreached = prefix_unreached;
+    A.rec = rec;
+    A.overflow = overflow;
+    A.box2 = box2;
+    A.box3 = box3;
+    A.valid = valid;
+    A.cand = cand;
+    A.reach = reach;
+    A.ctrl = ctrl;
+    A.rstep = rstep;
+    A.ctl = ctl;
+    A.chunk_states = sat_chunk;
+    A.sweep_all = opts.sweep_all_pairs ? 1u : 0u;
+    const uint32_t sat_blocks = blocks_for(S, sat_chunk);
+    const uint32_t max_it = cfg.max_iterations;
+    uint32_t done = 0;
+    uint32_t finished = 0;
+
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t first = 1; first <= max_it && !done; first += kBatch) {
+        const uint32_t last = std::min<uint64_t>(static_cast<uint64_t>(first) + kBatch - 1, max_it);
+        for (uint32_t it = first; it <= last; ++it) {
+            hipEvent_t* ev = &events[(it - first) * kEventsPerIter];
+            if (timed) GSC_HIP(hipEventRecord(ev[0], stream));
+            build_prefix(valid, reach, prefix_unreached, ctl);
+            if (timed) GSC_HIP(hipEventRecord(ev[1], stream));
+            A.iter = it;
+            hipLaunchKernelGGL(satisfy_kernel, dim3(sat_blocks), dim3(256), sat_shmem, stream, A);
+            GSC_HIP(hipGetLastError());
+            if (timed) GSC_HIP(hipEventRecord(ev[2], stream));
+            hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(64), 0, stream, ctl, records, it, max_it);
+            GSC_HIP(hipGetLastError());
+            if (timed) GSC_HIP(hipEventRecord(ev[3], stream));
+        }
+        GSC_HIP(hipMemcpyAsync(host_ctl, ctl, sizeof(IterCtl), hipMemcpyDeviceToHost, stream));
+        GSC_HIP(hipMemcpyAsync(host_records, records, sizeof(IterRecord) * kRing, hipMemcpyDeviceToHost, stream));
+        GSC_HIP(hipStreamSynchronize(stream));
+        done = host_ctl->done;
+        const uint32_t upto = host_ctl->iterations;
+        for (uint32_t it = first; it <= upto && it <= last; ++it) {
+            const IterRecord& r = host_records[it % kRing];
+            gsc_iteration_stats st;
+            std::memset(&st, 0, sizeof(st));
+            st.iteration = r.iteration;
+            st.newly_reachable = r.newly_reachable;
+            st.newly_certified = r.newly_certified;
+            st.total_reachable = r.total_reachable;
+            st.total_certified = r.total_certified;
+            if (timed) {
+                hipEvent_t* ev = &events[(it - first) * kEventsPerIter];
+                float a = 0, b = 0, c = 0, d = 0;
+                GSC_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+                GSC_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
+                GSC_HIP(hipEventElapsedTime(&c, ev[2], ev[3]));
+                GSC_HIP(hipEventElapsedTime(&d, ev[0], ev[3]));
+                st.prefix_build_ms = a;
+                st.satisfaction_check_ms = b;
+                st.reduction_ms = c;
+                st.iteration_ms = d;
+                prefix_ms += a;
+                sat_ms += b;
+                red_ms += c;
+            }
+            summary.pairs_tested += r.pairs_tested;
+            iter_stats.push_back(st);
+            if (opts.verbose && (it <= 5 || it % 10 == 0 || it == max_it))
+                std::printf("GPU: Iteration %u/%u, newly reachable: %llu, newly certified: %llu, total certified: "
+                            "%llu/%llu (%g ms) [prefix: %g ms, satisfaction: %g ms, reduction: %g ms]\n",
+                            it, max_it, r.newly_reachable, r.newly_certified, r.total_certified,
+                            static_cast<unsigned long long>(candidate_count), st.iteration_ms, st.prefix_build_ms,
+                            st.satisfaction_check_ms, st.reduction_ms);
+        }
+        finished = upto;
+    }
+    summary.solve_ms = ms_since(t0);
+    summary.iterations = finished;
+    summary.converged = done == kDoneConverged ? 1 : 0;
+    std::snprintf(summary.message, sizeof(summary.message), "%s", done_message(done));
+    summary.reachable_states = host_ctl->total_reachable;
+    summary.certified_candidate_states = host_ctl->total_certified;
+    summary.candidate_states = candidate_count;
+    summary.abstraction_ms = abstraction_kernel_ms;
+    summary.prefix_ms = prefix_ms;
+    summary.satisfaction_ms = sat_ms;
+    summary.reduction_ms = red_ms;
+    summary.pair_bytes_read = summary.pairs_tested * sizeof(uint32_t);
+    if (max_it == 0) {
+        summary.iterations = 0;
+        summary.converged = 0;
+        std::snprintf(summary.message, sizeof(summary.message), "%s", "maximum iterations reached");
+        summary.reachable_states = candidate_count;
+    }
+    timings = gsc_kernel_timings{};
+    timings.abstraction_kernel_ms = abstraction_kernel_ms;
+    timings.abstraction_memcpy_h2d_ms = setup_h2d_ms;
+    timings.prefix_build_total_ms = prefix_ms;
+    timings.pair_satisfaction_ms = sat_ms;
+    timings.reduce_inputs_ms = red_ms;
+    solved = true;
+}
+
+void gsc_solver::download(const gsc_result_buffers* out, gsc_run_report* report) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint64_t S = shape.state_count;
+    if (out) {
+        if (out->valid_mask) GSC_HIP(hipMemcpyAsync(out->valid_mask, valid, S, hipMemcpyDeviceToHost, stream));
+        if (out->candidate_mask) GSC_HIP(hipMemcpyAsync(out->candidate_mask, cand, S, hipMemcpyDeviceToHost, stream));
+        if (solved) {
+            if (out->reachable_mask)
+                GSC_HIP(hipMemcpyAsync(out->reachable_mask, reach, S, hipMemcpyDeviceToHost, stream));
+            if (out->controller)
+                GSC_HIP(hipMemcpyAsync(out->controller, ctrl, S * 4, hipMemcpyDeviceToHost, stream));
+            if (out->reach_step) GSC_HIP(hipMemcpyAsync(out->reach_step, rstep, S * 4, hipMemcpyDeviceToHost, stream));
+        }
+        GSC_HIP(hipStreamSynchronize(stream));
+        if (abstraction_ready && (out->abstraction_min_flat || out->abstraction_max_flat || out->abstraction_valid)) {
+            // Expand the records to the reference triplet in slices through device staging.
+            const uint64_t P = shape.pair_count;
+            const uint64_t slice = std::min<uint64_t>(P, 1ull << 26);
+            uint32_t* dmin = nullptr;
+            uint32_t* dmax = nullptr;
+            uint8_t* dval = nullptr;
+            GSC_HIP(hipMalloc(&dmin, slice * 4));
+            GSC_HIP(hipMalloc(&dmax, slice * 4));
+            GSC_HIP(hipMalloc(&dval, slice));
+            try {
+                for (uint64_t b = 0; b < P; b += slice) {
+                    const uint64_t e = std::min(P, b + slice);
+                    hipLaunchKernelGGL(export_kernel, dim3(blocks_for(e - b, 256)), dim3(256), 0, stream, dc, rec, box2,
+                                       box3, overflow, b, e, dmin, dmax, dval);
+                    GSC_HIP(hipGetLastError());
+                    if (out->abstraction_min_flat)
+                        GSC_HIP(hipMemcpyAsync(out->abstraction_min_flat + b, dmin, (e - b) * 4, hipMemcpyDeviceToHost,
+                                               stream));
+                    if (out->abstraction_max_flat)
+                        GSC_HIP(hipMemcpyAsync(out->abstraction_max_flat + b, dmax, (e - b) * 4, hipMemcpyDeviceToHost,
+                                               stream));
+                    if (out->abstraction_valid)
+                        GSC_HIP(hipMemcpyAsync(out->abstraction_valid + b, dval, e - b, hipMemcpyDeviceToHost, stream));
+                    GSC_HIP(hipStreamSynchronize(stream));
+                }
+            } catch (...) {
+                (void)hipFree(dmin);
+                (void)hipFree(dmax);
+                (void)hipFree(dval);
+                throw;
+            }
+            (void)hipFree(dmin);
+            (void)hipFree(dmax);
+            (void)hipFree(dval);
+        }
+        if (out->iteration_stats) {
+            const uint32_t n = std::min<uint32_t>(out->iteration_stats_capacity, static_cast<uint32_t>(iter_stats.size()));
+            std::memcpy(out->iteration_stats, iter_stats.data(), n * sizeof(gsc_iteration_stats));
+        }
+    }
+    timings.result_memcpy_d2h_ms = ms_since(t0);
+    timings.total_kernel_ms = timings.abstraction_kernel_ms + timings.pair_satisfaction_ms + timings.reduce_inputs_ms +
+                              timings.prefix_build_total_ms;
+    timings.total_memcpy_ms = timings.abstraction_memcpy_h2d_ms + timings.iteration_memcpy_ms + timings.result_memcpy_d2h_ms;
+    timings.total_compute_ms = timings.total_kernel_ms + timings.total_memcpy_ms;
+    if (report) {
+        report->compiled_with_gpu = 1;
+        report->runtime_available = 1;
+        report->executed = solved ? 1 : 0;
+        report->converged = summary.converged;
+        put(report->result_message, sizeof(report->result_message), summary.message);
+        report->abstraction_ms = abstraction_wall_ms;
+        report->solve_ms = summary.solve_ms;
+        report->total_ms = abstraction_wall_ms + summary.solve_ms;
+        report->iterations = summary.iterations;
+        report->iteration_stats_written =
+            out && out->iteration_stats ? std::min<uint32_t>(out->iteration_stats_capacity, static_cast<uint32_t>(iter_stats.size())) : 0;
+        report->reachable_states = summary.reachable_states;
+        report->certified_candidate_states = summary.certified_candidate_states;
+        report->candidate_states = candidate_count;
+        report->pair_count = shape.pair_count;
+        report->shape = shape;
+        report->budget = make_budget(shape);
+        report->kernel_timings = timings;
+        report->memory_stats.total_allocated_bytes = bytes_allocated;
+        report->memory_stats.abstraction_bytes = shape.pair_count * 4 + uint64_t(overflow_capacity) * sizeof(OverflowEntry);
+        report->memory_stats.prefix_bytes = shape.prefix_count * 4 * 2;
+        report->memory_stats.iteration_bytes = shape.state_count * (3 + 8) + sizeof(IterCtl) + sizeof(IterRecord) * kRing;
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) report->memory_stats.peak_usage_bytes = total_b - free_b;
+    }
+}
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+namespace {
+
+int status_of(const DeviceError& e) {
+    return e.code == hipErrorOutOfMemory || e.code == hipErrorMemoryAllocation ? GSC_ERR_OUT_OF_MEMORY : GSC_ERR_DEVICE;
+}
+
+void copy_case(gsc_solver* s, const gsc_case_desc& c) {
+    s->cfg = c;
+    s->name = c.name ? c.name : "";
+    s->obstacles.assign(c.obstacles, c.obstacles + (c.obstacles ? c.obstacle_count : 0));
+    s->targets.assign(c.extra_targets, c.extra_targets + (c.extra_targets ? c.extra_target_count : 0));
+    s->cfg.name = s->name.c_str();
+    s->cfg.obstacles = s->obstacles.empty() ? nullptr : s->obstacles.data();
+    s->cfg.obstacle_count = static_cast<uint32_t>(s->obstacles.size());
+    s->cfg.extra_targets = s->targets.empty() ? nullptr : s->targets.data();
+    s->cfg.extra_target_count = static_cast<uint32_t>(s->targets.size());
+}
+
+}  // namespace
+
+extern "C" {
+
+int gsc_backend_compiled(void) { return 1; }
+
+int gsc_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int gsc_case_shape_of(const gsc_case_desc* cfg, gsc_case_shape* out, char* err, size_t err_len) {
+    if (!cfg || !out) return GSC_ERR_INVALID_ARGUMENT;
+    try {
+        *out = make_shape(*cfg);
+        return GSC_OK;
+    } catch (const std::exception& e) {
+        put(err, err_len, e.what());
+        return GSC_ERR_CONFIG;
+    }
+}
+
+void gsc_solver_default_options(gsc_solver_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->device = 0;
+    o->keep_export = 0;
+    o->time_kernels = 1;
+    o->sweep_all_pairs = 0;
+    o->verbose = 0;
+}
+
+int gsc_solver_create(const gsc_case_desc* cfg, const gsc_solver_options* opts, gsc_solver** out, char* err,
+                      size_t err_len) {
+    if (!cfg || !out) return GSC_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    auto* s = new gsc_solver();
+    if (opts)
+        s->opts = *opts;
+    else
+        gsc_solver_default_options(&s->opts);
+    copy_case(s, *cfg);
+    try {
+        s->setup();
+    } catch (const ConfigError& e) {
+        put(err, err_len, e.what());
+        delete s;
+        return GSC_ERR_CONFIG;
+    } catch (const DeviceError& e) {
+        put(err, err_len, e.what());
+        const int st = status_of(e);
+        delete s;
+        return st;
+    } catch (const std::exception& e) {
+        put(err, err_len, e.what());
+        delete s;
+        return GSC_ERR_DEVICE;
+    }
+    *out = s;
+    return GSC_OK;
+}
+
+int gsc_solver_shape(const gsc_solver* s, gsc_case_shape* out) {
+    if (!s || !out) return GSC_ERR_INVALID_ARGUMENT;
+    *out = s->shape;
+    return GSC_OK;
+}
+
+#define GSC_GUARD(s, body)                                   \
+    try {                                                    \
+        body;                                                \
+        return GSC_OK;                                       \
+    } catch (const ConfigError& e) {                         \
+        (s)->last_error = e.what();                          \
+        return GSC_ERR_CONFIG;                               \
+    } catch (const DeviceError& e) {                         \
+        (s)->last_error = e.what();                          \
+        return status_of(e);                                 \
+    } catch (const std::exception& e) {                      \
+        (s)->last_error = e.what();                          \
+        return GSC_ERR_DEVICE;                               \
+    }
+
+int gsc_solver_build_abstraction(gsc_solver* s, double* kernel_ms) {
+    if (!s) return GSC_ERR_INVALID_ARGUMENT;
+    GSC_GUARD(s, {
+        s->build_abstraction();
+        if (kernel_ms) *kernel_ms = s->abstraction_kernel_ms;
+    })
+}
+
+int gsc_solver_solve(gsc_solver* s, gsc_solve_summary* out) {
+    if (!s) return GSC_ERR_INVALID_ARGUMENT;
+    GSC_GUARD(s, {
+        s->solve();
+        if (out) *out = s->summary;
+    })
+}
+
+int gsc_solver_download(gsc_solver* s, const gsc_result_buffers* out, gsc_run_report* report) {
+    if (!s) return GSC_ERR_INVALID_ARGUMENT;
+    GSC_GUARD(s, s->download(out, report))
+}
+
+const char* gsc_solver_last_error(const gsc_solver* s) { return s ? s->last_error.c_str() : ""; }
+
+void gsc_solver_destroy(gsc_solver* s) { delete s; }
+
+int gsc_run_case_u32(const gsc_case_desc* cfg, const gsc_result_buffers* out, gsc_run_report* report) {
+    if (!cfg || !report) return GSC_ERR_INVALID_ARGUMENT;
+    std::memset(report, 0, sizeof(*report));
+    report->compiled_with_gpu = 1;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        put(report->message, sizeof(report->message), "no HIP device found");
+        return GSC_ERR_NO_DEVICE;
+    }
+    report->runtime_available = 1;
+    gsc_solver_options opts;
+    gsc_solver_default_options(&opts);
+    opts.verbose = cfg->verbose;
+    gsc_solver* s = nullptr;
+    const int st = gsc_solver_create(cfg, &opts, &s, report->message, sizeof(report->message));
+    if (st != GSC_OK) return st;
+    int rc = gsc_solver_build_abstraction(s, nullptr);
+    if (rc == GSC_OK) rc = gsc_solver_solve(s, nullptr);
+    if (rc == GSC_OK) rc = gsc_solver_download(s, out, report);
+    if (rc != GSC_OK) {
+        put(report->message, sizeof(report->message), s->last_error);
+        report->executed = 0;
+    }
+    gsc_solver_destroy(s);
+    return rc;
+}
+
+}  // extern "C"

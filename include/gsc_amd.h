@@ -1,0 +1,304 @@
+/*
+ * gsc_amd.h — C ABI of the MI355X (gfx950) grid symbolic-control backend.
+ *
+ * The reference exposes its GPU backend as two C++ symbols,
+ *   bool gsc::cuda_backend_compiled();                       (include/gsc/cuda_api.hpp:169)
+ *   GpuRunReport gsc::run_case_cuda_u32(const CaseConfig&);  (include/gsc/cuda_api.hpp:170)
+ * whose arguments carry std::string / std::vector / std::function members
+ * (include/gsc/config.hpp:41-67). This header re-states that boundary as plain
+ * C: POD descriptors, caller-owned buffers, sizes and a status code, so that
+ * any FFI (ctypes, cgo, JNI) can bind it. The C++ mirror of the reference
+ * interface (gsc::run_case_cuda_u32 returning gsc::GpuRunReport) is layered on
+ * top of these entry points in gpu-inner-outer_amd/host/.
+ *
+ * Threading: every entry point is synchronous on the calling thread. Separate
+ * gsc_solver objects may be used from separate threads; one object must not be
+ * used from two threads at once.
+ */
+#ifndef GSC_AMD_H
+#define GSC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GSC_STATE_DIM 4
+#define GSC_INPUT_DIM 2
+/* Sentinels, include/gsc/types.hpp:22-23. */
+#define GSC_INVALID_INPUT 0xFFFFFFFFu
+#define GSC_UNREACHABLE_STEP 0xFFFFFFFFu
+#define GSC_MESSAGE_LEN 256
+
+typedef enum gsc_status {
+    GSC_OK = 0,
+    GSC_ERR_INVALID_ARGUMENT = 1, /* null pointer, bad size or buffer too small */
+    GSC_ERR_CONFIG = 2,           /* preparation failed; the reference throws these out of
+                                     run_case_cuda_u32 (kernels.cu:505, abstraction.hpp:261-265,
+                                     grid.hpp:146-167) */
+    GSC_ERR_NO_DEVICE = 3,        /* kernels.cu:496-501 "no CUDA device found" */
+    GSC_ERR_DEVICE = 4,           /* HIP runtime error (kernels.cu:21-27, caught at :856-858) */
+    GSC_ERR_OUT_OF_MEMORY = 5,    /* device allocation failed or HBM budget exceeded */
+    GSC_ERR_PARSE = 6,            /* .cfg parse error (config.cpp:243-347) */
+    GSC_ERR_BUSY = 7              /* serving queue full */
+} gsc_status;
+
+typedef enum gsc_constraint_type {
+    GSC_CONSTRAINT_NONE = 0,       /* config.hpp:19 */
+    GSC_CONSTRAINT_HYPERBOLIC = 1, /* config.hpp:20, config.cpp:64-68 */
+    GSC_CONSTRAINT_ELLIPTIC = 2    /* config.hpp:21, config.cpp:71-75 */
+} gsc_constraint_type;
+
+/* HyperRect4D, config.hpp:12-15 (closed intervals, config.cpp:79-86). */
+typedef struct gsc_rect4 {
+    double lb[GSC_STATE_DIM];
+    double ub[GSC_STATE_DIM];
+} gsc_rect4;
+
+/*
+ * CaseConfig, config.hpp:41-67, as a POD. Arrays are borrowed for the duration
+ * of a call. The three trailing extension fields have no reference
+ * counterpart; left zero they reproduce the reference exactly:
+ *   extra_targets   : further target boxes; a state is a candidate if its
+ *                     centre lies in `candidate` OR in any extra target box.
+ *   target_band_*   : when enabled, a centre in `candidate` only counts if
+ *                     fabs(x*y - c) <= w as well (a hyperbolic band in (x, y)).
+ */
+typedef struct gsc_case_desc {
+    const char* name;
+    double state_lb[GSC_STATE_DIM];
+    double state_ub[GSC_STATE_DIM];
+    double state_eta[GSC_STATE_DIM];
+    int32_t wrap_dim;
+    double input_lb[GSC_INPUT_DIM];
+    double input_ub[GSC_INPUT_DIM];
+    double input_eta[GSC_INPUT_DIM];
+    gsc_rect4 candidate;
+    gsc_rect4 map;
+    const gsc_rect4* obstacles;
+    uint32_t obstacle_count;
+    int32_t constraint_type; /* gsc_constraint_type */
+    double hyperbolic_a, hyperbolic_b, hyperbolic_c;
+    double elliptic_a, elliptic_b;
+    double disturbance_half_width[GSC_STATE_DIM];
+    double sample_time;
+    uint32_t integration_substeps;
+    uint32_t max_iterations;
+    int32_t verbose;
+    /* extensions */
+    const gsc_rect4* extra_targets;
+    uint32_t extra_target_count;
+    int32_t target_band_enabled;
+    double target_band_c;
+    double target_band_w;
+} gsc_case_desc;
+
+/* Grid4D / InputGrid2D / PrefixLayout4D sizes (grid.hpp:42-131, 133-198, 283-299). */
+typedef struct gsc_case_shape {
+    uint32_t state_shape[GSC_STATE_DIM];
+    uint64_t state_strides[GSC_STATE_DIM];
+    uint64_t state_count;
+    uint32_t input_shape[GSC_INPUT_DIM];
+    uint64_t input_strides[GSC_INPUT_DIM];
+    uint64_t input_count;
+    uint64_t pair_count;
+    uint32_t prefix_shape[GSC_STATE_DIM];
+    uint64_t prefix_count;
+} gsc_case_shape;
+
+/* IterationStats, cuda_api.hpp:11-21. */
+typedef struct gsc_iteration_stats {
+    uint32_t iteration;
+    uint32_t reserved;
+    uint64_t newly_reachable;
+    uint64_t newly_certified;
+    uint64_t total_reachable;
+    uint64_t total_certified;
+    double iteration_ms;
+    double prefix_build_ms;
+    double satisfaction_check_ms;
+    double reduction_ms;
+} gsc_iteration_stats;
+
+/* KernelTimings, cuda_api.hpp:24-49 (same field names and meaning). */
+typedef struct gsc_kernel_timings {
+    double abstraction_kernel_ms;
+    double abstraction_memcpy_h2d_ms;
+    double scatter_mask_ms;
+    double scan_axis0_ms;
+    double scan_axis1_ms;
+    double scan_axis2_ms;
+    double scan_axis3_ms;
+    double prefix_build_total_ms;
+    double pair_satisfaction_ms;
+    double reduce_inputs_ms;
+    double iteration_memcpy_ms;
+    double result_memcpy_d2h_ms;
+    double total_kernel_ms;
+    double total_memcpy_ms;
+    double total_compute_ms;
+} gsc_kernel_timings;
+
+/* GpuMemoryStats, cuda_api.hpp:52-59. */
+typedef struct gsc_memory_stats {
+    uint64_t total_allocated_bytes;
+    uint64_t peak_usage_bytes;
+    uint64_t abstraction_bytes;
+    uint64_t prefix_bytes;
+    uint64_t iteration_bytes;
+    uint64_t result_bytes;
+} gsc_memory_stats;
+
+/* MemoryBudget, abstraction.hpp:16-30 (the reference's estimate, u64 prefix tables). */
+typedef struct gsc_memory_budget {
+    uint64_t pair_min_bytes;
+    uint64_t pair_max_bytes;
+    uint64_t pair_valid_bytes;
+    uint64_t pair_satisfied_bytes;
+    uint64_t reachable_prev_bytes;
+    uint64_t reachable_next_bytes;
+    uint64_t valid_mask_bytes;
+    uint64_t candidate_mask_bytes;
+    uint64_t controller_bytes;
+    uint64_t reach_step_bytes;
+    uint64_t prefix_reachable_bytes;
+    uint64_t prefix_valid_bytes;
+    uint64_t total_bytes;
+} gsc_memory_budget;
+
+/* The scalar part of GpuRunReport, cuda_api.hpp:61-85. */
+typedef struct gsc_run_report {
+    int32_t compiled_with_gpu;
+    int32_t runtime_available;
+    int32_t executed;
+    int32_t converged;
+    char message[GSC_MESSAGE_LEN];        /* report.message (errors) */
+    char result_message[GSC_MESSAGE_LEN]; /* report.result.message (stop reason) */
+    double abstraction_ms;
+    double solve_ms;
+    double total_ms;
+    uint32_t iterations;
+    uint32_t iteration_stats_written;
+    uint64_t reachable_states;
+    uint64_t certified_candidate_states;
+    uint64_t candidate_states;
+    uint64_t pair_count;
+    gsc_case_shape shape;
+    gsc_memory_budget budget;
+    gsc_kernel_timings kernel_timings;
+    gsc_memory_stats memory_stats;
+} gsc_run_report;
+
+/*
+ * Host buffers filled by a run (any pointer may be NULL to skip that output).
+ * State arrays hold state_count entries, abstraction arrays pair_count
+ * entries, iteration_stats iteration_stats_capacity entries. The abstraction
+ * triplet is the reference layout (kernels.cu:806-824): min_flat/max_flat of
+ * the successor cell box and valid, all zero for an invalid pair.
+ */
+typedef struct gsc_result_buffers {
+    uint8_t* valid_mask;
+    uint8_t* candidate_mask;
+    uint8_t* reachable_mask;
+    uint32_t* controller;
+    uint32_t* reach_step;
+    uint32_t* abstraction_min_flat;
+    uint32_t* abstraction_max_flat;
+    uint8_t* abstraction_valid;
+    gsc_iteration_stats* iteration_stats;
+    uint32_t iteration_stats_capacity;
+} gsc_result_buffers;
+
+/* Replaces gsc::cuda_backend_compiled (cuda_api.hpp:169, kernels.cu:488-490). Returns 1. */
+int gsc_backend_compiled(void);
+
+/* Number of visible HIP devices (kernels.cu:496-497); 0 when none or on error. */
+int gsc_device_count(void);
+
+/*
+ * Grid shapes of a case (build_state_grid / build_input_grid / build_prefix_layout,
+ * grid.hpp:133-198, 283-299). Fails with GSC_ERR_CONFIG and the reference's
+ * message for non-positive eta, ub <= lb or a state count above 2^32-1.
+ */
+int gsc_case_shape_of(const gsc_case_desc* cfg, gsc_case_shape* out, char* err, size_t err_len);
+
+/*
+ * Replaces gsc::run_case_cuda_u32 (cuda_api.hpp:170, kernels.cu:492-875): build the
+ * masks, the pair abstraction, then iterate the backward fixpoint to convergence,
+ * stall or max_iterations, and copy the requested results to `out`.
+ * Returns GSC_OK with report->executed = 1 on success. Preparation failures the
+ * reference lets escape as exceptions return GSC_ERR_CONFIG; device errors the
+ * reference catches into report.message return GSC_ERR_DEVICE/OUT_OF_MEMORY with
+ * report->executed = 0. Every device buffer is released before returning.
+ */
+int gsc_run_case_u32(const gsc_case_desc* cfg, const gsc_result_buffers* out, gsc_run_report* report);
+
+/* ---------------------------------------------------------------------------
+ * Device-resident solver. No reference counterpart: the reference allocates and
+ * frees everything per call (kernels.cu:513-560, 860-873). These entry points
+ * keep masks, abstraction and iteration state in HBM so a caller can re-solve,
+ * time stages separately, or read back only what it needs.
+ * ------------------------------------------------------------------------- */
+typedef struct gsc_solver gsc_solver;
+
+typedef struct gsc_solver_options {
+    int32_t device;          /* HIP device ordinal */
+    int32_t keep_export;     /* 1: also keep the reference abstraction triplet on the device */
+    int32_t time_kernels;    /* 1: record per-kernel HIP events (per-iteration stats) */
+    int32_t sweep_all_pairs; /* 1: test every pair every iteration (the reference's work
+                                schedule); 0: test pairs of unsettled states only */
+    int32_t verbose;         /* print per-iteration progress like kernels.cu:746-756 */
+    int32_t reserved[3];
+} gsc_solver_options;
+
+typedef struct gsc_solve_summary {
+    uint32_t iterations;
+    int32_t converged;
+    uint64_t reachable_states;
+    uint64_t certified_candidate_states;
+    uint64_t candidate_states;
+    double abstraction_ms;       /* abstraction kernel(s), HIP events */
+    double solve_ms;             /* host wall time of the fixpoint loop */
+    double prefix_ms;            /* sum of prefix-build kernel time (HIP events) */
+    double satisfaction_ms;      /* sum of pair-satisfaction kernel time (HIP events) */
+    double reduction_ms;         /* sum of the per-iteration bookkeeping kernel time */
+    uint64_t pairs_tested;       /* pairs whose successor box was queried, all iterations */
+    uint64_t pair_bytes_read;    /* algorithmic bytes of pair records streamed, all iterations */
+    char message[GSC_MESSAGE_LEN];
+} gsc_solve_summary;
+
+void gsc_solver_default_options(gsc_solver_options* opts);
+int gsc_solver_create(const gsc_case_desc* cfg, const gsc_solver_options* opts,
+                      gsc_solver** out, char* err, size_t err_len);
+int gsc_solver_shape(const gsc_solver* s, gsc_case_shape* out);
+/* Compute the pair abstraction on the device (kernels.cu:164-217 equivalent). */
+int gsc_solver_build_abstraction(gsc_solver* s, double* kernel_ms);
+/* Reset the iteration state to the seeds (kernels.cu:576-598) and run the fixpoint. */
+int gsc_solver_solve(gsc_solver* s, gsc_solve_summary* out);
+/* Copy results (and, with keep_export, the abstraction triplet) to host buffers. */
+int gsc_solver_download(gsc_solver* s, const gsc_result_buffers* out, gsc_run_report* report);
+/* Last error text of this solver. */
+const char* gsc_solver_last_error(const gsc_solver* s);
+void gsc_solver_destroy(gsc_solver* s);
+
+/* ---------------------------------------------------------------------------
+ * .cfg parsing (config.cpp:243-347), kept so the CLI and FFI callers read the
+ * same files as the reference. Extension keys: `target` (8 numbers, repeatable)
+ * and `target_band` (c, w). Returns a handle owning the descriptor and its arrays.
+ * ------------------------------------------------------------------------- */
+typedef struct gsc_config gsc_config;
+int gsc_config_load(const char* path, gsc_config** out, char* err, size_t err_len);
+int gsc_config_parse(const char* text, gsc_config** out, char* err, size_t err_len);
+const gsc_case_desc* gsc_config_desc(const gsc_config* c);
+void gsc_config_free(gsc_config* c);
+/* make_default_case / make_wraparound_case / make_smoke_case / make_hyperbolic_case
+ * (config.cpp:129-241); which = 0..3. */
+int gsc_config_builtin(int which, gsc_config** out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSC_AMD_H */

@@ -1,0 +1,99 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Bit-exact for everything (integer sets, counts, tables and the FP64-derived
+cell boxes): masks, abstraction triplets, reachable/candidate sets,
+controller and reach_step tables, per-iteration counts, stop message.
+"""
+import numpy as np
+import pytest
+
+from casegen import assert_same_abstraction, assert_same_solution, small_random_case
+from gsc_amd import cases
+from gsc_amd.solver import GscError, Solver, run_case_u32
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(case, oracle, abstraction=True, label=None):
+    label = label or case.name
+    print(f"[case] {label} states={case.state_count()} inputs={case.input_count()}", flush=True)
+    ref = oracle.run(case)
+    got = run_case_u32(case, abstraction=abstraction)
+    assert got.executed, got.message
+    assert_same_solution(got, ref, label)
+    if abstraction:
+        assert_same_abstraction(got, ref, label)
+    return got, ref
+
+
+@pytest.mark.parametrize("which", ["default", "wraparound", "smoke"])
+def test_builtin_cases(gpu, oracle, which):
+    # tests/test_small_case.cpp:48-56 runs the default and wraparound cases CPU vs GPU.
+    _check(cases.builtin(which), oracle)
+
+
+def test_hyperbolic_builtin_first_iterations(gpu, oracle):
+    c = cases.builtin("hyperbolic")
+    c.max_iterations = 3
+    _check(c, oracle)
+
+
+@pytest.mark.parametrize("T", [0.1, 0.5])
+def test_unicycle_30x30x24x10(gpu, oracle, T):
+    _check(cases.unicycle_30x30x24x10(sample_time=T), oracle)
+
+
+@pytest.mark.parametrize("seed", list(range(24)))
+def test_small_random(gpu, oracle, seed):
+    c = small_random_case(seed)
+    try:
+        ref_masks = oracle.masks(c)
+    except ValueError as e:
+        with pytest.raises(GscError) as ei:
+            run_case_u32(c)
+        assert str(ei.value) == str(e)
+        return
+    _check(c, oracle)
+
+
+def test_sweep_all_pairs_schedule(gpu, oracle):
+    # The reference's schedule (every pair tested every iteration) must give the same result.
+    c = cases.unicycle_30x30x24x10(sample_time=0.5)
+    ref = oracle.run(c)
+    with Solver(c, sweep_all_pairs=True) as s:
+        s.build_abstraction()
+        summ = s.solve()
+        got = s.download()
+    assert_same_solution(got, ref, "sweep")
+    assert summ["pairs_tested"] == c.state_count() * c.input_count() * summ["iterations"]
+
+
+def test_resolve_is_idempotent(gpu, oracle):
+    # Re-seeding and re-solving on resident state gives the same answer twice.
+    c = cases.unicycle_30x30x24x10(sample_time=0.5)
+    ref = oracle.run(c)
+    with Solver(c) as s:
+        s.build_abstraction()
+        s.solve()
+        a = s.download()
+        s.solve()
+        b = s.download()
+    assert_same_solution(a, ref, "first")
+    assert_same_solution(b, ref, "second")
+
+
+def test_candidate_outside_valid_raises(gpu):
+    c = cases.builtin("default")
+    # Cell centres sit at ±0.25: this obstacle removes four candidate columns.
+    c.obstacles = [((-0.3, -0.3, -4.0, 0.0), (0.3, 0.3, 4.0, 2.0))]
+    with pytest.raises(GscError, match="candidate set contains invalid states"):
+        run_case_u32(c)
+
+
+def test_max_iterations_zero(gpu, oracle):
+    c = cases.unicycle_30x30x24x10(sample_time=0.5)
+    c.max_iterations = 0
+    ref = oracle.run(c)
+    got = run_case_u32(c)
+    assert_same_solution(got, ref, "max0")
+    assert got.result_message == "maximum iterations reached"
