@@ -113,7 +113,7 @@ def main():
         s = solver.solve()
         print(f"[bench] warmup {w}: {s['iterations']} iterations, {s['message']}", file=sys.stderr, flush=True)
 
-    abs_ms, solve_ms, sat_ms, prefix_ms, red_ms, iters, tested = 0.0, 0.0, 0.0, 0.0, 0.0, 0, 0
+    abs_ms, solve_ms, sat_ms, prefix_ms, red_ms, iters, tested, counted, slots = 0.0, 0.0, 0.0, 0.0, 0.0, 0, 0, 0, 0
     t0 = time.perf_counter()
     for k in range(args.steps):
         a = solver.build_abstraction()
@@ -125,6 +125,8 @@ def main():
         red_ms += s["reduction_ms"]
         iters += s["iterations"]
         tested += s["pairs_tested"]
+        counted += s["boxes_counted"]
+        slots += s["test_slots"]
     wall_ms = (time.perf_counter() - t0) * 1000.0
     summary = s
     K = args.steps
@@ -176,6 +178,8 @@ def main():
         "prefix_ms_per_iteration": prefix_ms / iters,
         "bookkeeping_ms_per_iteration": red_ms / iters,
         "pairs_tested_per_iteration": tested / iters,
+        "boxes_counted_per_iteration": counted / iters,
+        "test_lane_utilisation": tested / max(1, slots),
         "reference_published_rtx4090_gpairs": REF_PUBLISHED_GPAIRS,
         "roofline": roof,
         "cpu_baseline": None,

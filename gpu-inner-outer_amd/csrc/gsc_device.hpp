@@ -33,8 +33,10 @@ constexpr double kEps = 1e-9;  // grid.hpp:216
 //   bits 25-29  y extent
 //   bit  30     dead: the box holds no valid cell, so the pair can never be
 //               satisfied (the reference's `valid_count == 0` test,
-//               reachability.hpp:387-394 / kernels.cu:327-332, hoisted: the
-//               valid set never changes)
+//               reachability.hpp:387-394 / kernels.cu:327-332). The valid set
+//               never changes, so the answer is memoised: the abstraction
+//               writes 0 ("not known dead") and the satisfaction pass sets the
+//               bit the first time it counts zero valid cells in the box.
 //   bit  31     escape: bits 0-30 index an overflow entry holding the full
 //               (min_flat, max_flat) pair; used for boxes that do not fit the
 //               fields above (wrapped x/y ranges, large boxes, inverted ranges)
@@ -78,6 +80,13 @@ struct DevCase {
     uint32_t pstride[4];  // padded prefix strides (grid.hpp:283-299)
     uint32_t S;           // states
     uint32_t I;           // inputs
+    uint32_t I_pad;       // inputs rounded up to a whole number of satisfaction blocks;
+                          // records of the padding inputs are invalid
+    uint32_t PQ;          // cells per (θ, v) plane: n_x * n_y
+    uint32_t chunks_per_plane;  // ceil(PQ / 64): 64-cell wave tasks per plane
+    uint32_t n_tasks;     // planes * chunks_per_plane
+    uint32_t groups_per_plane;  // ceil(PQ / 256): satisfaction workgroups per plane
+    uint32_t n_groups;    // planes * groups_per_plane
     uint32_t n_a;         // input shape[0]: a (dv = u[0], model.hpp:39)
     uint32_t n_w;         // input shape[1]: ω (dθ = u[1], model.hpp:38)
     uint32_t K;           // integration substeps

@@ -25,11 +25,6 @@ namespace {
 
 __device__ inline uint32_t lane_id() { return threadIdx.x & 63u; }
 
-__device__ inline uint64_t lanes_below_mask() {
-    const uint32_t l = lane_id();
-    return l == 0 ? 0ull : (~0ull >> (64u - l));
-}
-
 // Inclusive prefix sum across a 64-lane wave.
 __device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
     const uint32_t l = lane_id();
@@ -45,6 +40,16 @@ __device__ inline uint64_t wave_sum_u64(uint64_t v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
+}
+
+// Pointer into the constant address space: loads through it at wave-uniform
+// addresses compile to scalar (s_load) instructions.
+template <typename T>
+using ConstPtr = const __attribute__((address_space(4))) T*;
+
+template <typename T>
+__device__ inline ConstPtr<T> to_const(const T* p) {
+    return reinterpret_cast<ConstPtr<T>>(reinterpret_cast<uintptr_t>(p));
 }
 
 __device__ inline bool loop_done(const IterCtl* ctl) {
@@ -127,11 +132,13 @@ __global__ void reset_ctl_kernel(IterCtl* ctl, unsigned long long candidate_coun
         ctl->total_reachable = candidate_count;
         ctl->total_certified = 0;
     }
-    for (uint32_t k = t; k < kRing; k += blockDim.x) {
-        ctl->ring[k].new_reachable = 0;
-        ctl->ring[k].new_certified = 0;
-        ctl->ring[k].pairs_tested = 0;
-        ctl->ring[k].pad = 0;
+    for (uint32_t k = t; k < kRing * kCounterSlots; k += blockDim.x) {
+        const uint32_t r = k / kCounterSlots, j = k % kCounterSlots;
+        ctl->ring[r].new_reachable[j] = 0;
+        ctl->ring[r].new_certified[j] = 0;
+        ctl->ring[r].pairs_tested[j] = 0;
+        ctl->ring[r].boxes_counted[j] = 0;
+        ctl->ring[r].test_slots[j] = 0;
     }
 }
 
@@ -161,6 +168,18 @@ __global__ __launch_bounds__(256) void prefix_rows_kernel(DevCase c, const uint8
         if (x < c.shape[0]) P[dst + x] = v;
         carry = __shfl(v, 63, 64);
     }
+}
+
+// Bitmask of valid, unreached states (bit s of word s / 64), for the probe in
+// satisfy_kernel. Thread per state; lane 0 of each wave stores the wave's ballot.
+__global__ __launch_bounds__(256) void unreached_bits_kernel(uint32_t S, const uint8_t* __restrict__ valid,
+                                                              const uint8_t* __restrict__ reach,
+                                                              unsigned long long* __restrict__ bits, const IterCtl* ctl) {
+    if (ctl && loop_done(ctl)) return;
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool u = s < S && valid[s] != 0 && reach[s] == 0;
+    const uint64_t b = __ballot(u);
+    if (lane_id() == 0 && s < S) bits[s >> 6] = b;
 }
 
 // Pass 2..4: running sum along `axis` (1, 2 or 3); one thread per line, lines
@@ -204,74 +223,139 @@ __global__ __launch_bounds__(256) void prefix_axis_kernel(DevCase c, int axis, u
 }
 
 // ---------------------------------------------------------------------------
-// Abstraction. A workgroup owns a tile of 64 consecutive states; lane l of
-// every wave is state s0+l, and the four waves take interleaved inputs, so the
-// trajectory tables are read at wave-uniform addresses. Records are staged in
-// LDS as [state][input] and written out as one contiguous block.
-//
-// Per pair, the arithmetic is the reference's in the reference's order:
+// Pair records are stored plane-major: rec[(plane * I + input) * PQ + q] with
+// plane = θ cell + n_θ * v cell and q = x + n_x * y (PQ = n_x * n_y cells per
+// plane). A wave owns 64 consecutive cells q of one plane and walks the
+// inputs in order, so
+//   - record loads and stores are 64 consecutive words,
+//   - the θ / v cell ranges (box2, box3) and the trajectory tables are
+//     wave-uniform (scalar loads),
+//   - the 16 prefix corners of neighbouring lanes are neighbouring words.
+__device__ inline void wave_task(const DevCase& c, uint32_t task, uint32_t* plane, uint32_t* q0) {
+    *plane = task / c.chunks_per_plane;
+    *q0 = (task - *plane * c.chunks_per_plane) * 64u;
+}
+
+// Per-step position increments of the nominal trajectory, model.hpp:36-42:
+//   inc[(plane * I + input) * K + k] = (dt*(v_k*cos θ_k), dt*(v_k*sin θ_k)).
+// They depend on (θ cell, v cell, input) only, so they are computed once here
+// and every pair of a plane adds the same increments to its own x0, y0 — the
+// reference's `out[0] += dt * dx` with identical operands and rounding.
+__global__ __launch_bounds__(256) void increments_kernel(DevCase c, const double2* __restrict__ cs,
+                                                          const double* __restrict__ vt, double2* __restrict__ inc) {
+    const uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint64_t total = static_cast<uint64_t>(c.shape[2]) * c.shape[3] * c.I * c.K;
+    if (t >= total) return;
+    const uint32_t k = static_cast<uint32_t>(t % c.K);
+    const uint64_t r = t / c.K;
+    const uint32_t i = static_cast<uint32_t>(r % c.I);
+    const uint32_t plane = static_cast<uint32_t>(r / c.I);
+    const uint32_t iz = plane % c.shape[2];
+    const uint32_t iv = plane / c.shape[2];
+    const uint32_t iw = i / c.n_a;
+    const uint32_t ia = i - iw * c.n_a;
+    const double2 t2 = cs[static_cast<size_t>(iz * c.n_w + iw) * c.K + k];
+    const double vk = vt[static_cast<size_t>(iv * c.n_a + ia) * c.K + k];
+    const double dx = vk * t2.x;
+    const double dy = vk * t2.y;
+    inc[t] = double2{c.dt * dx, c.dt * dy};
+}
+
+// θ/v part of the probe cell per (plane, input): lo2 * stride2 + lo3 * stride3 of
+// the pair's successor box (0 for padding inputs and rejected ranges, whose
+// records are invalid and never probed).
+__global__ __launch_bounds__(256) void probe_zw_kernel(DevCase c, const uint2* __restrict__ box2,
+                                                        const uint2* __restrict__ box3, uint32_t* __restrict__ probe_zw) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = c.shape[2] * c.shape[3] * c.I_pad;
+    if (t >= n) return;
+    const uint32_t plane = t / c.I_pad;
+    const uint32_t i = t - plane * c.I_pad;
+    uint32_t v = 0;
+    if (i < c.I) {
+        const uint32_t iw = i / c.n_a;
+        const uint2 b2 = box2[(plane % c.shape[2]) * c.n_w + iw];
+        const uint2 b3 = box3[(plane / c.shape[2]) * c.n_a + (i - iw * c.n_a)];
+        if (b2.x != kBoxReject && b3.x != kBoxReject) v = b2.x * c.stride[2] + b3.x * c.stride[3];
+    }
+    probe_zw[t] = v;
+}
+
+// Abstraction. Per pair, the arithmetic is the reference's in the reference's order:
 //   centre           grid.hpp:72
-//   nominal x, y     model.hpp:35-42: x += dt*(v_k*cos θ_k), y += dt*(v_k*sin θ_k)
-//                    with cos θ_k, sin θ_k, v_k from host tables
+//   nominal x, y     model.hpp:35-42: x += dt*(v_k*cos θ_k), y += dt*(v_k*sin θ_k),
+//                    the increments read from increments_kernel's table
 //   radius           model.hpp:71-73, 84-89, 107-111 (per v cell, host table)
 //   cells            grid.hpp:211-281 (dims 0,1 here; dims 2,3 from box2/box3)
 //   constraint scan  abstraction.hpp:110-159 as a 2D prefix query over x,y fails
+// The `dead` bit of compact records is left 0 here and settled lazily by the
+// satisfaction pass (see satisfy_kernel); escaped pairs get it eagerly.
 __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
-    extern __shared__ uint32_t tile[];
     const DevCase& c = A.c;
     const uint32_t lane = lane_id();
-    const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t s0 = blockIdx.x * 64u;
-    const uint32_t s = s0 + lane;
-    const bool live = s < c.S;
-    const uint32_t n_live = min(64u, c.S - s0);
-
-    uint32_t idx[4] = {0, 0, 0, 0};
-    if (live) unflatten(c, s, idx);
-    const double x0 = cell_center(c.lb[0], idx[0], c.eta[0]);
-    const double y0 = cell_center(c.lb[1], idx[1], c.eta[1]);
+    const uint32_t task = blockIdx.x * 4u + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (task >= c.n_tasks) return;
+    uint32_t plane, q0;
+    wave_task(c, task, &plane, &q0);
+    const uint32_t iz = plane % c.shape[2];
+    const uint32_t iv = plane / c.shape[2];
+    const uint32_t q = q0 + lane;
+    const bool live = q < c.PQ;
+    const uint32_t iy = live ? q / c.shape[0] : 0;
+    const uint32_t ix = live ? q - iy * c.shape[0] : 0;
+    const double x0 = cell_center(c.lb[0], ix, c.eta[0]);
+    const double y0 = cell_center(c.lb[1], iy, c.eta[1]);
     const uint32_t fpitch = c.shape[0] + 1;
     bool state_ok = live;
     if (live && c.constraint_type != GSC_CONSTRAINT_NONE) {
         // abstraction.hpp:177-181: the state's own centre must satisfy the constraint.
         const uint32_t* F = A.fail2d;
-        const uint32_t f = F[(idx[1] + 1) * fpitch + idx[0] + 1] - F[(idx[1] + 1) * fpitch + idx[0]] -
-                           F[idx[1] * fpitch + idx[0] + 1] + F[idx[1] * fpitch + idx[0]];
+        const uint32_t f = F[(iy + 1) * fpitch + ix + 1] - F[(iy + 1) * fpitch + ix] - F[iy * fpitch + ix + 1] +
+                           F[iy * fpitch + ix];
         state_ok = f == 0;
     }
-    const double rad0 = A.rad[2 * idx[3]];
-    const double rad1 = A.rad[2 * idx[3] + 1];
+    const double rad0 = A.rad[2 * iv];
+    const double rad1 = A.rad[2 * iv + 1];
     const int wd = c.wrap_dim;
-
-    for (uint32_t i0 = 0; i0 < c.I; i0 += A.in_chunk) {
-        const uint32_t ic = min(A.in_chunk, c.I - i0);
-        for (uint32_t ii = wave; ii < ic; ii += 4) {
-            const uint32_t i = i0 + ii;
+    const uint64_t base = static_cast<uint64_t>(plane) * c.I_pad * c.PQ + q;
+    if (live)
+        for (uint32_t i = c.I; i < c.I_pad; ++i) A.rec[base + static_cast<uint64_t>(i) * c.PQ] = kRecInvalid;
+    uint32_t i = 0;
+    for (uint32_t iw = 0; iw < c.n_w; ++iw) {
+        const uint2 b2 = A.box2[iz * c.n_w + iw];
+        for (uint32_t ia = 0; ia < c.n_a; ++ia, ++i) {
+            const uint2 b3 = A.box3[iv * c.n_a + ia];
             uint32_t r = kRecInvalid;
             uint32_t lo[4], hi[4];
-            bool ok = state_ok;
-            const uint32_t ia = i % c.n_a;
-            const uint32_t iw = i / c.n_a;
+            bool ok = state_ok && b2.x != kBoxReject && b3.x != kBoxReject;
             if (ok) {
-                const uint2 b2 = A.box2[idx[2] * c.n_w + iw];
-                const uint2 b3 = A.box3[idx[3] * c.n_a + ia];
-                ok = b2.x != kBoxReject && b3.x != kBoxReject;
                 lo[2] = b2.x;
                 hi[2] = b2.y;
                 lo[3] = b3.x;
                 hi[3] = b3.y;
-            }
-            if (ok) {
-                const double2* __restrict__ cs = A.cs + static_cast<size_t>(idx[2] * c.n_w + iw) * c.K;
-                const double* __restrict__ vt = A.vt + static_cast<size_t>(idx[3] * c.n_a + ia) * c.K;
+                // The increments are wave-uniform: read them through the constant
+                // address space (scalar loads), eight steps per wait. The additions
+                // stay strictly in step order.
+                const ConstPtr<double> inc =
+                    to_const(reinterpret_cast<const double*>(A.inc + (static_cast<size_t>(plane) * c.I + i) * c.K));
                 double px = x0, py = y0;
-                for (uint32_t k = 0; k < c.K; ++k) {
-                    const double2 t = cs[k];
-                    const double vk = vt[k];
-                    const double dx = vk * t.x;
-                    const double dy = vk * t.y;
-                    px = px + c.dt * dx;
-                    py = py + c.dt * dy;
+                uint32_t k = 0;
+                for (; k + 8 <= c.K; k += 8) {
+                    double tx[8], ty[8];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        tx[j] = inc[2 * (k + j)];
+                        ty[j] = inc[2 * (k + j) + 1];
+                    }
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        px = px + tx[j];
+                        py = py + ty[j];
+                    }
+                }
+                for (; k < c.K; ++k) {
+                    px = px + inc[2 * k];
+                    py = py + inc[2 * k + 1];
                 }
                 ok = dim_to_cells(c, 0, px - rad0, px + rad0, &lo[0], &hi[0]) &&
                      dim_to_cells(c, 1, py - rad1, py + rad1, &lo[1], &hi[1]);
@@ -294,13 +378,12 @@ __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
                 // Inverted ranges on a non-wrap axis or wrapped x/y ranges take the escape path.
                 const bool regular = lo[0] <= hi[0] && lo[1] <= hi[1] && (lo[2] <= hi[2] || wd == 2) &&
                                      (lo[3] <= hi[3] || wd == 3);
-                const int off0 = static_cast<int>(lo[0]) - static_cast<int>(idx[0]);
-                const int off1 = static_cast<int>(lo[1]) - static_cast<int>(idx[1]);
+                const int off0 = static_cast<int>(lo[0]) - static_cast<int>(ix);
+                const int off1 = static_cast<int>(lo[1]) - static_cast<int>(iy);
                 const bool fits = regular && off0 >= kOffMin && off0 <= kOffMax && off1 >= kOffMin &&
                                   off1 <= kOffMax && hi[0] - lo[0] <= kExtMax && hi[1] - lo[1] <= kExtMax;
                 if (fits) {
-                    const uint32_t vc = box_count_wrap_u32(A.prefix_valid, c.pstride, wd, c.shape[wd], lo, hi);
-                    r = rec_pack(off0, hi[0] - lo[0], off1, hi[1] - lo[1], vc == 0);
+                    r = rec_pack(off0, hi[0] - lo[0], off1, hi[1] - lo[1], false);
                 } else {
                     const int64_t vc = box_count_i64(A.prefix_valid, c.pstride, wd, c.shape[wd], lo, hi);
                     const uint32_t e = atomicAdd(A.overflow_count, 1u);
@@ -317,26 +400,13 @@ __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
                     }
                 }
             }
-            if (live) tile[lane * ic + ii] = r;
+            if (live) A.rec[base + static_cast<uint64_t>(i) * c.PQ] = r;
         }
-        __syncthreads();
-        const uint64_t base = static_cast<uint64_t>(s0) * c.I + i0;
-        if (ic == c.I) {
-            const uint32_t n = n_live * ic;
-            for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) A.rec[base + t] = tile[t];
-        } else {
-            const uint32_t n = n_live * ic;
-            for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) {
-                const uint32_t l = t / ic;
-                A.rec[base + static_cast<uint64_t>(l) * c.I + (t - l * ic)] = tile[t];
-            }
-        }
-        __syncthreads();
     }
 }
 
-// Reference abstraction triplet (min_flat, max_flat, valid) from the records;
-// zeros for invalid pairs (kernels.cu:189-216). Thread per pair.
+// Reference abstraction triplet (min_flat, max_flat, valid) in the reference's
+// pair order p = state * I + input; zeros for invalid pairs (kernels.cu:189-216).
 __global__ __launch_bounds__(256) void export_kernel(DevCase c, const uint32_t* __restrict__ rec,
                                                       const uint2* __restrict__ box2, const uint2* __restrict__ box3,
                                                       const OverflowEntry* __restrict__ overflow, uint64_t pair_begin,
@@ -344,191 +414,291 @@ __global__ __launch_bounds__(256) void export_kernel(DevCase c, const uint32_t* 
                                                       uint32_t* __restrict__ out_max, uint8_t* __restrict__ out_valid) {
     const uint64_t p = pair_begin + static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= pair_end) return;
-    const uint64_t q = p - pair_begin;
-    const uint32_t r = rec[p];
+    const uint64_t o = p - pair_begin;
+    const uint32_t s = static_cast<uint32_t>(p / c.I);
+    const uint32_t i = static_cast<uint32_t>(p - static_cast<uint64_t>(s) * c.I);
+    const uint32_t plane = s / c.PQ;
+    const uint32_t q = s - plane * c.PQ;
+    const uint32_t r = rec[(static_cast<uint64_t>(plane) * c.I_pad + i) * c.PQ + q];
     if (r == kRecInvalid) {
-        out_min[q] = 0;
-        out_max[q] = 0;
-        out_valid[q] = 0;
+        out_min[o] = 0;
+        out_max[o] = 0;
+        out_valid[o] = 0;
         return;
     }
     if (r & kRecEscape) {
-        const OverflowEntry o = overflow[r & kRecIndexMask];
-        out_min[q] = o.min_flat;
-        out_max[q] = o.max_flat;
-        out_valid[q] = 1;
+        const OverflowEntry e = overflow[r & kRecIndexMask];
+        out_min[o] = e.min_flat;
+        out_max[o] = e.max_flat;
+        out_valid[o] = 1;
         return;
     }
-    const uint32_t s = static_cast<uint32_t>(p / c.I);
-    const uint32_t i = static_cast<uint32_t>(p - static_cast<uint64_t>(s) * c.I);
-    uint32_t idx[4];
-    unflatten(c, s, idx);
-    const uint2 b2 = box2[idx[2] * c.n_w + i / c.n_a];
-    const uint2 b3 = box3[idx[3] * c.n_a + i % c.n_a];
-    const uint32_t lo0 = idx[0] + rec_off0(r), lo1 = idx[1] + rec_off1(r);
-    out_min[q] = lo0 * c.stride[0] + lo1 * c.stride[1] + b2.x * c.stride[2] + b3.x * c.stride[3];
-    out_max[q] = (lo0 + rec_ext0(r)) * c.stride[0] + (lo1 + rec_ext1(r)) * c.stride[1] + b2.y * c.stride[2] +
+    const uint32_t iy = q / c.shape[0];
+    const uint32_t ix = q - iy * c.shape[0];
+    const uint32_t iz = plane % c.shape[2];
+    const uint32_t iv = plane / c.shape[2];
+    const uint2 b2 = box2[iz * c.n_w + i / c.n_a];
+    const uint2 b3 = box3[iv * c.n_a + i % c.n_a];
+    const uint32_t lo0 = ix + rec_off0(r), lo1 = iy + rec_off1(r);
+    out_min[o] = lo0 * c.stride[0] + lo1 * c.stride[1] + b2.x * c.stride[2] + b3.x * c.stride[3];
+    out_max[o] = (lo0 + rec_ext0(r)) * c.stride[0] + (lo1 + rec_ext1(r)) * c.stride[1] + b2.y * c.stride[2] +
                  b3.y * c.stride[3];
-    out_valid[q] = 1;
+    out_valid[o] = 1;
 }
 
 // ---------------------------------------------------------------------------
-// Pair satisfaction + first-input reduction, one workgroup per chunk of
-// consecutive states.
-//   1. load state flags, compact the states to test into LDS (wave ballots);
-//   2. test their pairs in flat (state, input) order — coalesced 4-byte record
-//      loads — with `sat = !dead && count(unreached valid cells in box) == 0`,
-//      which equals the reference's `valid && cntV > 0 && cntR == cntV`
-//      because reachable ⊆ valid; each wave stores its 64-pair ballot as one
-//      word of an LDS bitmap;
-//   3. per unsettled state, the lowest set bit of its input range is the
-//      chosen input (reachability.hpp:106 / kernels.cu:369-374); update the
-//      state in place (safe: the prefix snapshot was built before this launch,
-//      and only this workgroup touches these states).
+// Pair satisfaction, the rare slow part: a full prefix count of the box of one
+// pair that the bitmask probe could not rule out (or an escaped pair).
+//
+// sat = !dead && count(unreached valid cells in the box) == 0, which equals the
+// reference's `valid && cntV > 0 && cntR == cntV` because reachable ⊆ valid
+// makes cntV - cntR the count of valid, unreached cells.
+__device__ inline bool count_test(const SatisfyArgs& A, uint32_t r, uint32_t ix, uint32_t iy, uint2 b2, uint2 b3,
+                                        uint32_t* rec_slot) {
+    const DevCase& c = A.c;
+    const int wd = c.wrap_dim;
+    if (r & kRecEscape) {
+        const OverflowEntry o = A.overflow[r & kRecIndexMask];
+        if (o.dead) return false;
+        uint32_t lo[4], hi[4];
+        unflatten(c, o.min_flat, lo);
+        unflatten(c, o.max_flat, hi);
+        return box_count_i64(A.prefix_unreached, c.pstride, wd, c.shape[wd], lo, hi) == 0;
+    }
+    uint32_t lo[4], hi[4];
+    lo[0] = ix + rec_off0(r);
+    hi[0] = lo[0] + rec_ext0(r);
+    lo[1] = iy + rec_off1(r);
+    hi[1] = lo[1] + rec_ext1(r);
+    lo[2] = b2.x;
+    hi[2] = b2.y;
+    lo[3] = b3.x;
+    hi[3] = b3.y;
+    if (box_count_wrap_u32(A.prefix_unreached, c.pstride, wd, c.shape[wd], lo, hi) != 0) return false;
+    // No unreached valid cell: satisfied unless the box has no valid cell at all;
+    // that answer never changes, so it is memoised in the record.
+    if (box_count_wrap_u32(A.prefix_valid, c.pstride, wd, c.shape[wd], lo, hi) == 0) {
+        *rec_slot = r | kRecDead;
+        return false;
+    }
+    return true;
+}
+
+// Pair satisfaction fused with the first-input reduction.
+//
+// A workgroup owns a group of 256 consecutive cells of one (θ, v) plane. It
+// first compacts the states that need testing into LDS, so each wave runs 64
+// of them (no lanes spent on settled or invalid states). Each wave then walks
+// the inputs in increasing order in blocks of kInputBlock: the block's record
+// loads are issued together, then a bitmask probe per pair reads the whole
+// x-row of the successor box at its middle y (θ and v at their lower ends) —
+// any valid unreached cell there proves the pair unsatisfied — and only the
+// pairs the probe cannot rule out get the full prefix count. A lane stops at
+// its first satisfied input and the wave when no lane is left (the reference
+// breaks at the first satisfied input, reachability.hpp:106 /
+// kernels.cu:369-374). With sweep_all every pair of every state is tested, the
+// reference kernel's schedule (kernels.cu:668).
+// States are updated in place: the prefix snapshot was built before this
+// launch and only this workgroup touches these states (Jacobi, as kernels.cu:742).
 __global__ __launch_bounds__(256) void satisfy_kernel(SatisfyArgs A) {
     if (loop_done(A.ctl)) return;
     const DevCase& c = A.c;
-    extern __shared__ uint64_t bits[];
-    __shared__ uint32_t s_state[256];
-    __shared__ uint32_t s_idx[4][256];
-    __shared__ uint8_t s_use[256];
-    __shared__ uint32_t s_wave_count[4];
-    __shared__ unsigned long long s_counts[3];
-
+    constexpr uint32_t kUse = 1, kCand = 2, kReached = 4, kCertified = 8;
+    __shared__ uint32_t s_q[256];
+    __shared__ uint8_t s_flags[256];
+    __shared__ uint32_t s_wave_n[4];
+    __shared__ uint32_t s_counts[4][5];
     const uint32_t lane = lane_id();
-    const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t s0 = blockIdx.x * A.chunk_states;
-    const uint32_t t = threadIdx.x;
-    const uint32_t s = s0 + t;
-    if (t < 3) s_counts[t] = 0;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t plane = blockIdx.x / c.groups_per_plane;
+    const uint32_t g0 = (blockIdx.x - plane * c.groups_per_plane) * 256u;
+    const uint32_t PQ = c.PQ;
 
-    bool use = false, eval = false;
-    uint32_t idx[4] = {0, 0, 0, 0};
-    if (t < A.chunk_states && s < c.S) {
-        if (A.valid[s]) {
-            const bool is_cand = A.cand[s] != 0;
-            const bool reached = A.reach[s] != 0;
-            const bool certified = A.ctrl[s] != GSC_INVALID_INPUT;
-            use = !((!is_cand && reached) || (is_cand && certified));
+    // 1. flags of the group's cells, compacted in LDS (cell order kept).
+    {
+        const uint32_t q = g0 + threadIdx.x;
+        const uint32_t s = plane * PQ + q;
+        uint32_t f = 0;
+        bool eval = false;
+        if (q < PQ) {
+            if (A.valid[s]) {
+                const bool is_cand = A.cand[s] != 0;
+                const bool reached = A.reach[s] != 0;
+                const bool certified = A.ctrl[s] != GSC_INVALID_INPUT;
+                const bool use = !((!is_cand && reached) || (is_cand && certified));
+                f = (use ? kUse : 0) | (is_cand ? kCand : 0) | (reached ? kReached : 0) | (certified ? kCertified : 0);
+            }
+            eval = A.sweep_all ? true : (f & kUse) != 0;
         }
-        eval = A.sweep_all ? true : use;
-        if (eval) unflatten(c, s, idx);
+        const uint64_t m = __ballot(eval);
+        if (lane == 0) s_wave_n[wave] = static_cast<uint32_t>(__popcll(m));
+        __syncthreads();
+        uint32_t base = 0;
+        for (uint32_t w = 0; w < wave; ++w) base += s_wave_n[w];
+        if (eval) {
+            const uint32_t pos = base + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
+            s_q[pos] = q;
+            s_flags[pos] = static_cast<uint8_t>(f);
+        }
+        __syncthreads();
     }
-    const uint64_t m = __ballot(eval);
-    if (lane == 0) s_wave_count[wave] = static_cast<uint32_t>(__popcll(m));
-    __syncthreads();
-    uint32_t base_pos = 0;
-    for (uint32_t w = 0; w < wave; ++w) base_pos += s_wave_count[w];
-    const uint32_t n_eval = s_wave_count[0] + s_wave_count[1] + s_wave_count[2] + s_wave_count[3];
-    if (eval) {
-        const uint32_t pos = base_pos + static_cast<uint32_t>(__popcll(m & lanes_below_mask()));
-        s_state[pos] = s;
-        s_use[pos] = use ? 1 : 0;
-        s_idx[0][pos] = idx[0];
-        s_idx[1][pos] = idx[1];
-        s_idx[2][pos] = idx[2];
-        s_idx[3][pos] = idx[3];
-    }
-    __syncthreads();
-    if (n_eval == 0) return;
+    const uint32_t n_eval = s_wave_n[0] + s_wave_n[1] + s_wave_n[2] + s_wave_n[3];
 
-    const uint32_t I = c.I;
-    const uint32_t n_pairs = n_eval * I;
-    const int wd = c.wrap_dim;
-    for (uint32_t pbase = wave * 64u; pbase < n_pairs; pbase += blockDim.x) {
-        const uint32_t q = pbase + lane;
-        bool sat = false;
-        if (q < n_pairs) {
-            const uint32_t a = q / I;
-            const uint32_t i = q - a * I;
-            const uint32_t st = s_state[a];
-            const uint32_t r = A.rec[static_cast<uint64_t>(st) * I + i];
-            if (r != kRecInvalid) {
-                if (r & kRecEscape) {
-                    const OverflowEntry o = A.overflow[r & kRecIndexMask];
-                    if (!o.dead) {
-                        uint32_t lo[4], hi[4];
-                        unflatten(c, o.min_flat, lo);
-                        unflatten(c, o.max_flat, hi);
-                        sat = box_count_i64(A.prefix_unreached, c.pstride, wd, c.shape[wd], lo, hi) == 0;
+    // 2. each wave tests 64 of the compacted states.
+    uint32_t nr = 0, nc = 0, tested = 0, counted = 0, slots = 0;
+    const uint32_t e = wave * 64u + lane;
+    const bool has = e < n_eval;
+    if (wave * 64u < n_eval) {
+        const uint32_t q = has ? s_q[e] : g0;
+        const uint32_t flags = has ? s_flags[e] : 0u;
+        const uint32_t s = plane * PQ + q;
+        const uint32_t iy = q / c.shape[0];
+        const uint32_t ix = q - iy * c.shape[0];
+        const uint32_t lane_cell = ix + iy * c.stride[1];
+        const uint32_t st1 = c.stride[1];
+        // Everything indexed by (plane, input) is wave-uniform: the record slice
+        // base (32-bit lane offsets from it) and the probe's θ/v cell part.
+        uint32_t* const rec_plane = A.rec + static_cast<uint64_t>(plane) * c.I_pad * PQ;
+        const ConstPtr<uint32_t> probe_zw = to_const(A.probe_zw + static_cast<size_t>(plane) * c.I_pad);
+        const uint32_t* const ubits = reinterpret_cast<const uint32_t*>(A.unreached_bits);
+        uint32_t found = GSC_INVALID_INPUT;
+        bool pending = has;
+        for (uint32_t i0 = 0; i0 < c.I_pad; i0 += kInputBlock) {
+            const uint32_t nb = min(kInputBlock, c.I - i0);  // real (non-padding) inputs in the block
+            uint32_t r[kInputBlock];
+#pragma unroll
+            for (uint32_t j = 0; j < kInputBlock; ++j) r[j] = kRecInvalid;
+            if (pending) {
+                tested += nb;
+#pragma unroll
+                for (uint32_t j = 0; j < kInputBlock; ++j) r[j] = rec_plane[(i0 + j) * PQ + q];
+            }
+            if (lane == 0) slots += 64u * nb;
+            // Row probe: bits [row, row + ext0] of the unreached bitmask, the box's
+            // x-range at its middle y and lower θ, v (inside the box also when θ
+            // wraps). Escaped, dead and invalid records are not probed.
+            uint32_t row[kInputBlock], w0[kInputBlock], w1[kInputBlock];
+#pragma unroll
+            for (uint32_t j = 0; j < kInputBlock; ++j) {
+                const uint32_t rj = r[j];
+                const bool compact = (rj & (kRecEscape | kRecDead)) == 0;
+                const uint32_t px = lane_cell + rec_off0(rj);
+                const uint32_t py = (rec_off1(rj) + (rec_ext1(rj) >> 1)) * st1;
+                row[j] = compact ? px + py + probe_zw[i0 + j] : 0u;
+                w0[j] = ubits[row[j] >> 5];
+                w1[j] = ubits[(row[j] >> 5) + 1];
+            }
+            // Which pairs need the full count: a compact record the probe could
+            // not rule out, or an escaped record (never an invalid or dead one).
+            uint32_t need_mask = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < kInputBlock; ++j) {
+                const uint32_t rj = r[j];
+                const bool compact = (rj & (kRecEscape | kRecDead)) == 0;
+                const uint64_t window = ((static_cast<uint64_t>(w1[j]) << 32) | w0[j]) >> (row[j] & 31u);
+                const uint64_t span = (2ull << rec_ext0(rj)) - 1ull;  // ext0 + 1 low bits
+                const bool blocked = compact && (window & span) != 0;
+                if (!blocked && rj != kRecInvalid && !(rj & kRecDead)) need_mask |= 1u << j;
+            }
+            // The count itself is rare: one non-unrolled pass over the block, in input
+            // order, so the first satisfied input wins. Records are re-read here
+            // rather than kept selectable in registers.
+            if (__ballot(need_mask != 0) != 0) {
+#pragma unroll 1
+                for (uint32_t j = 0; j < kInputBlock; ++j) {
+                    const bool need = ((need_mask >> j) & 1u) && pending;
+                    if (__ballot(need) == 0) continue;
+                    if (need) {
+                        uint32_t* const slot = rec_plane + (i0 + j) * PQ + q;
+                        const uint32_t i = i0 + j;
+                        const uint32_t iw = i / c.n_a;
+                        const uint2 b2 = A.box2[(plane % c.shape[2]) * c.n_w + iw];
+                        const uint2 b3 = A.box3[(plane / c.shape[2]) * c.n_a + (i - iw * c.n_a)];
+                        ++counted;
+                        if (count_test(A, *slot, ix, iy, b2, b3, slot)) {
+                            if (found == GSC_INVALID_INPUT) found = i;
+                            if (!A.sweep_all) pending = false;
+                        }
                     }
-                } else if (!(r & kRecDead)) {
-                    const uint2 b2 = A.box2[s_idx[2][a] * c.n_w + i / c.n_a];
-                    const uint2 b3 = A.box3[s_idx[3][a] * c.n_a + i % c.n_a];
-                    uint32_t lo[4], hi[4];
-                    lo[0] = s_idx[0][a] + rec_off0(r);
-                    hi[0] = lo[0] + rec_ext0(r);
-                    lo[1] = s_idx[1][a] + rec_off1(r);
-                    hi[1] = lo[1] + rec_ext1(r);
-                    lo[2] = b2.x;
-                    hi[2] = b2.y;
-                    lo[3] = b3.x;
-                    hi[3] = b3.y;
-                    sat = box_count_wrap_u32(A.prefix_unreached, c.pstride, wd, c.shape[wd], lo, hi) == 0;
                 }
             }
+            if (!A.sweep_all && __ballot(pending) == 0) break;
         }
-        const uint64_t b = __ballot(sat);
-        if (lane == 0) bits[pbase >> 6] = b;
-    }
-    __syncthreads();
-
-    uint64_t nr = 0, nc = 0;
-    if (t < n_eval && s_use[t]) {
-        const uint32_t first = t * I;
-        const uint32_t last = first + I;  // exclusive
-        uint32_t chosen = GSC_INVALID_INPUT;
-        for (uint32_t wpos = first >> 6; (wpos << 6) < last; ++wpos) {
-            uint64_t word = bits[wpos];
-            const uint32_t wstart = wpos << 6;
-            if (wstart < first) word &= ~0ull << (first - wstart);
-            if (wstart + 64 > last) word &= ~0ull >> (wstart + 64 - last);
-            if (word) {
-                chosen = wstart + static_cast<uint32_t>(__ffsll(static_cast<long long>(word)) - 1) - first;
-                break;
-            }
-        }
-        if (chosen != GSC_INVALID_INPUT) {
-            const uint32_t st = s_state[t];
-            const bool is_cand = A.cand[st] != 0;
-            const bool reached = A.reach[st] != 0;
-            const bool certified = A.ctrl[st] != GSC_INVALID_INPUT;
-            if (!reached) {
-                A.reach[st] = 1;
+        // 3. update the state (reachability.hpp:401-415, kernels.cu:376-392).
+        if ((flags & kUse) && found != GSC_INVALID_INPUT) {
+            if (!(flags & kReached)) {
+                A.reach[s] = 1;
                 nr = 1;
             }
-            if (!certified) {
-                A.ctrl[st] = chosen;
-                if (is_cand) {
+            if (!(flags & kCertified)) {
+                A.ctrl[s] = found;
+                if (flags & kCand) {
                     nc = 1;
-                } else if (A.rstep[st] == GSC_UNREACHABLE_STEP) {
-                    A.rstep[st] = A.iter;
+                } else if (A.rstep[s] == GSC_UNREACHABLE_STEP) {
+                    A.rstep[s] = A.iter;
                 }
             }
         }
     }
-    nr = wave_sum_u64(nr);
-    nc = wave_sum_u64(nc);
+    // Per-wave sums (32-bit is ample: a wave tests at most 64 * I pairs).
+    for (int off = 32; off > 0; off >>= 1) {
+        nr += __shfl_xor(nr, off, 64);
+        nc += __shfl_xor(nc, off, 64);
+        tested += __shfl_xor(tested, off, 64);
+        counted += __shfl_xor(counted, off, 64);
+    }
     if (lane == 0) {
-        if (nr) atomicAdd(&s_counts[0], static_cast<unsigned long long>(nr));
-        if (nc) atomicAdd(&s_counts[1], static_cast<unsigned long long>(nc));
+        s_counts[wave][0] = nr;
+        s_counts[wave][1] = nc;
+        s_counts[wave][2] = tested;
+        s_counts[wave][3] = counted;
+        s_counts[wave][4] = slots;
     }
     __syncthreads();
-    if (t == 0) {
+    if (threadIdx.x == 0) {
+        unsigned long long a = 0, b = 0, t = 0, n = 0, sl = 0;
+        for (int w = 0; w < 4; ++w) {
+            a += s_counts[w][0];
+            b += s_counts[w][1];
+            t += s_counts[w][2];
+            n += s_counts[w][3];
+            sl += s_counts[w][4];
+        }
         IterCounters& slot = A.ctl->ring[A.iter % kRing];
-        if (s_counts[0]) atomicAdd(&slot.new_reachable, s_counts[0]);
-        if (s_counts[1]) atomicAdd(&slot.new_certified, s_counts[1]);
-        atomicAdd(&slot.pairs_tested, static_cast<unsigned long long>(n_pairs));
+        const uint32_t k = blockIdx.x % kCounterSlots;
+        if (a) atomicAdd(&slot.new_reachable[k], a);
+        if (b) atomicAdd(&slot.new_certified[k], b);
+        if (t) atomicAdd(&slot.pairs_tested[k], t);
+        if (n) atomicAdd(&slot.boxes_counted[k], n);
+        if (sl) atomicAdd(&slot.test_slots[k], sl);
     }
 }
 
-// Totals and the stop test of one iteration (kernels.cu:722-766). One thread.
+// Totals and the stop test of one iteration (kernels.cu:722-766). One wave.
 __global__ void finish_kernel(IterCtl* ctl, IterRecord* records, uint32_t iter, uint32_t max_iterations) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (ctl->done) return;
+    if (blockIdx.x != 0 || threadIdx.x >= 64) return;
+    if (loop_done(ctl)) return;
     IterCounters& slot = ctl->ring[iter % kRing];
-    const unsigned long long nr = slot.new_reachable;
-    const unsigned long long nc = slot.new_certified;
+    const uint32_t l = threadIdx.x;
+    unsigned long long nr = 0, nc = 0, pt = 0, bc = 0, ts = 0;
+    if (l < kCounterSlots) {
+        nr = slot.new_reachable[l];
+        nc = slot.new_certified[l];
+        pt = slot.pairs_tested[l];
+        bc = slot.boxes_counted[l];
+        ts = slot.test_slots[l];
+        slot.new_reachable[l] = 0;
+        slot.new_certified[l] = 0;
+        slot.pairs_tested[l] = 0;
+        slot.boxes_counted[l] = 0;
+        slot.test_slots[l] = 0;
+    }
+    nr = wave_sum_u64(nr);
+    nc = wave_sum_u64(nc);
+    pt = wave_sum_u64(pt);
+    bc = wave_sum_u64(bc);
+    ts = wave_sum_u64(ts);
+    if (l != 0) return;
     ctl->total_reachable += nr;
     ctl->total_certified += nc;
     ctl->iterations = iter;
@@ -538,10 +708,9 @@ __global__ void finish_kernel(IterCtl* ctl, IterRecord* records, uint32_t iter, 
     rec.newly_certified = nc;
     rec.total_reachable = ctl->total_reachable;
     rec.total_certified = ctl->total_certified;
-    rec.pairs_tested = slot.pairs_tested;
-    slot.new_reachable = 0;
-    slot.new_certified = 0;
-    slot.pairs_tested = 0;
+    rec.pairs_tested = pt;
+    rec.boxes_counted = bc;
+    rec.test_slots = ts;
     uint32_t done = 0;
     if (ctl->total_certified == ctl->candidate_count)
         done = kDoneConverged;

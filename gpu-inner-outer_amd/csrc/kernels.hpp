@@ -13,11 +13,18 @@ constexpr uint32_t kDoneConverged = 1;      // "all candidate states obtained a 
 constexpr uint32_t kDoneStalled = 2;        // "iteration stalled before candidate closure"
 constexpr uint32_t kDoneMaxIterations = 3;  // "maximum iterations reached"
 
+// Workgroups add into one of kCounterSlots slots (by block index) so that tens
+// of thousands of workgroups do not serialise on a single address.
+constexpr uint32_t kCounterSlots = 32;
+// Inputs judged per block in satisfy_kernel (loads of a block are issued together).
+constexpr uint32_t kInputBlock = 8;
+
 struct IterCounters {
-    unsigned long long new_reachable;
-    unsigned long long new_certified;
-    unsigned long long pairs_tested;
-    unsigned long long pad;
+    unsigned long long new_reachable[kCounterSlots];
+    unsigned long long new_certified[kCounterSlots];
+    unsigned long long pairs_tested[kCounterSlots];
+    unsigned long long boxes_counted[kCounterSlots];
+    unsigned long long test_slots[kCounterSlots];  // lane slots of executed test blocks
 };
 
 struct IterCtl {
@@ -37,12 +44,13 @@ struct IterRecord {
     unsigned long long total_reachable;
     unsigned long long total_certified;
     unsigned long long pairs_tested;
+    unsigned long long boxes_counted;
+    unsigned long long test_slots;
 };
 
 struct AbstractionArgs {
     DevCase c;
-    const double2* cs;           // [n_θ][n_ω][K] (cos θ_k, sin θ_k)
-    const double* vt;            // [n_v][n_a][K] v_k
+    const double2* inc;          // [plane][input][K] nominal-trajectory increments
     const uint2* box2;           // [n_θ][n_ω] θ cell range (or kBoxReject)
     const uint2* box3;           // [n_v][n_a] v cell range (or kBoxReject)
     const double* rad;           // [n_v][2] x / y box radius
@@ -52,13 +60,15 @@ struct AbstractionArgs {
     OverflowEntry* overflow;
     uint32_t* overflow_count;
     uint32_t overflow_capacity;
-    uint32_t in_chunk;
 };
 
 struct SatisfyArgs {
     DevCase c;
     const uint32_t* prefix_unreached;
-    const uint32_t* rec;
+    const uint32_t* prefix_valid;
+    const unsigned long long* unreached_bits;
+    const uint32_t* probe_zw;  // [plane][I_pad]: lo2*stride2 + lo3*stride3 of the pair's box
+    uint32_t* rec;
     const OverflowEntry* overflow;
     const uint2* box2;
     const uint2* box3;
@@ -69,7 +79,6 @@ struct SatisfyArgs {
     uint32_t* rstep;
     IterCtl* ctl;
     uint32_t iter;
-    uint32_t chunk_states;
     uint32_t sweep_all;
 };
 
@@ -79,6 +88,10 @@ __global__ void seed_kernel(uint32_t S, const uint8_t* cand, uint8_t* reach, uin
 __global__ void reset_ctl_kernel(IterCtl* ctl, unsigned long long candidate_count);
 __global__ void prefix_rows_kernel(DevCase c, const uint8_t* a, const uint8_t* b, uint32_t* P, const IterCtl* ctl);
 __global__ void prefix_axis_kernel(DevCase c, int axis, uint32_t* P, const IterCtl* ctl);
+__global__ void increments_kernel(DevCase c, const double2* cs, const double* vt, double2* inc);
+__global__ void unreached_bits_kernel(uint32_t S, const uint8_t* valid, const uint8_t* reach, unsigned long long* bits,
+                                      const IterCtl* ctl);
+__global__ void probe_zw_kernel(DevCase c, const uint2* box2, const uint2* box3, uint32_t* probe_zw);
 __global__ void abstraction_kernel(AbstractionArgs A);
 __global__ void export_kernel(DevCase c, const uint32_t* rec, const uint2* box2, const uint2* box3,
                               const OverflowEntry* overflow, uint64_t pair_begin, uint64_t pair_end,

@@ -147,6 +147,9 @@ struct gsc_solver {
     uint32_t* overflow_count = nullptr;
     double2* cs = nullptr;
     double* vt = nullptr;
+    double2* inc = nullptr;
+    unsigned long long* ubits = nullptr;
+    uint32_t* probe_zw = nullptr;
     uint2 *box2 = nullptr, *box3 = nullptr;
     double* rad = nullptr;
     uint32_t* fail2d = nullptr;
@@ -161,9 +164,6 @@ struct gsc_solver {
     uint64_t candidate_count = 0;
     bool abstraction_ready = false;
     bool solved = false;
-    uint32_t in_chunk = 0;
-    uint32_t sat_chunk = 0;
-    size_t sat_shmem = 0;
 
     double abstraction_kernel_ms = 0.0;
     double abstraction_wall_ms = 0.0;
@@ -303,6 +303,18 @@ void gsc_solver::setup() {
     dc.wrap_span_minus_eps = (cfg.state_ub[cfg.wrap_dim] - cfg.state_lb[cfg.wrap_dim]) - kEps;
     dc.S = static_cast<uint32_t>(shape.state_count);
     dc.I = static_cast<uint32_t>(shape.input_count);
+    dc.I_pad = (dc.I + kInputBlock - 1) / kInputBlock * kInputBlock;
+    dc.PQ = shape.state_shape[0] * shape.state_shape[1];
+    if (static_cast<uint64_t>(dc.I_pad) * dc.PQ >= 0xFFFFFFFFull)
+        throw ConfigError("input count x plane size exceeds the 32-bit record offset");
+    dc.chunks_per_plane = (dc.PQ + 63) / 64;
+    {
+        const uint64_t tasks = static_cast<uint64_t>(dc.chunks_per_plane) * shape.state_shape[2] * shape.state_shape[3];
+        if (tasks * 64 >= 0xFFFFFFFFull) throw ConfigError("state grid too large for one launch");
+        dc.n_tasks = static_cast<uint32_t>(tasks);
+        dc.groups_per_plane = (dc.PQ + 255) / 256;
+        dc.n_groups = dc.groups_per_plane * shape.state_shape[2] * shape.state_shape[3];
+    }
     dc.n_a = shape.input_shape[0];
     dc.n_w = shape.input_shape[1];
     dc.K = cfg.integration_substeps;
@@ -323,7 +335,7 @@ void gsc_solver::setup() {
 
     GSC_HIP(hipSetDevice(opts.device));
     GSC_HIP(hipStreamCreate(&stream));
-    const uint64_t S = shape.state_count, P = shape.pair_count, PT = shape.prefix_count;
+    const uint64_t S = shape.state_count, PT = shape.prefix_count;
     valid = alloc<uint8_t>(S);
     cand = alloc<uint8_t>(S);
     reach = alloc<uint8_t>(S);
@@ -331,7 +343,10 @@ void gsc_solver::setup() {
     rstep = alloc<uint32_t>(S);
     prefix_valid = alloc<uint32_t>(PT);
     prefix_unreached = alloc<uint32_t>(PT);
-    rec = alloc<uint32_t>(P);
+    const uint64_t n_planes = static_cast<uint64_t>(shape.state_shape[2]) * shape.state_shape[3];
+    rec = alloc<uint32_t>(n_planes * dc.I_pad * dc.PQ);
+    probe_zw = alloc<uint32_t>(n_planes * dc.I_pad);
+    ubits = alloc<unsigned long long>((S + 63) / 64 + 1);  // +1: the row probe reads one word past
     overflow_capacity = 1u << 16;
     overflow = alloc<OverflowEntry>(overflow_capacity);
     overflow_count = alloc<uint32_t>(1);
@@ -356,6 +371,9 @@ void gsc_solver::setup() {
     box3 = alloc<uint2>(h_box3.size());
     rad = alloc<double>(h_rad.size());
     if (!h_fail.empty()) fail2d = alloc<uint32_t>(h_fail.size());
+    const uint64_t n_inc = static_cast<uint64_t>(dc.shape[2]) * dc.shape[3] * dc.I * dc.K;
+    if (n_inc * sizeof(double2) > (4ull << 30)) throw ConfigError("trajectory increment table exceeds 4 GiB");
+    inc = alloc<double2>(n_inc);
     d_obstacles = alloc<gsc_rect4>(obstacles.size());
     d_targets = alloc<gsc_rect4>(targets.size());
 
@@ -378,6 +396,13 @@ void gsc_solver::setup() {
     GSC_HIP(hipMemsetAsync(prefix_unreached, 0, PT * sizeof(uint32_t), stream));
     GSC_HIP(hipMemsetAsync(mask_counts, 0, 2 * sizeof(unsigned long long), stream));
 
+    hipLaunchKernelGGL(probe_zw_kernel, dim3(blocks_for(n_planes * dc.I_pad, 256)), dim3(256), 0, stream, dc, box2,
+                       box3, probe_zw);
+    GSC_HIP(hipGetLastError());
+    if (n_inc) {
+        hipLaunchKernelGGL(increments_kernel, dim3(blocks_for(n_inc, 256)), dim3(256), 0, stream, dc, cs, vt, inc);
+        GSC_HIP(hipGetLastError());
+    }
     // Masks on the device (the reference builds them in a serial host loop even in
     // GPU mode, abstraction.hpp:257-265).
     hipLaunchKernelGGL(masks_kernel, dim3(blocks_for(S, 256)), dim3(256), 0, stream, dc, d_obstacles, d_targets, valid,
@@ -392,11 +417,6 @@ void gsc_solver::setup() {
 
     build_prefix(valid, nullptr, prefix_valid, nullptr);
 
-    in_chunk = std::min<uint32_t>(dc.I, 128u);
-    const uint32_t max_bits = 65536u * 8u;
-    sat_chunk = std::min<uint32_t>(256u, max_bits / dc.I);
-    if (sat_chunk == 0) throw ConfigError("input grid too large");
-    sat_shmem = ((static_cast<size_t>(sat_chunk) * dc.I + 63) / 64) * sizeof(uint64_t);
 }
 
 void gsc_solver::build_prefix(const uint8_t* a, const uint8_t* b, uint32_t* P, const IterCtl* c) {
@@ -418,8 +438,7 @@ void gsc_solver::build_abstraction() {
         GSC_HIP(hipMemsetAsync(overflow_count, 0, sizeof(uint32_t), stream));
         AbstractionArgs A{};
         A.c = dc;
-        A.cs = cs;
-        A.vt = vt;
+        A.inc = inc;
         A.box2 = box2;
         A.box3 = box3;
         A.rad = rad;
@@ -429,10 +448,8 @@ void gsc_solver::build_abstraction() {
         A.overflow = overflow;
         A.overflow_count = overflow_count;
         A.overflow_capacity = overflow_capacity;
-        A.in_chunk = in_chunk;
-        const size_t shmem = static_cast<size_t>(64) * in_chunk * sizeof(uint32_t);
         GSC_HIP(hipEventRecord(events[0], stream));
-        hipLaunchKernelGGL(abstraction_kernel, dim3(blocks_for(dc.S, 64)), dim3(256), shmem, stream, A);
+        hipLaunchKernelGGL(abstraction_kernel, dim3(blocks_for(dc.n_tasks, 4)), dim3(256), 0, stream, A);
         GSC_HIP(hipGetLastError());
         GSC_HIP(hipEventRecord(events[1], stream));
         uint32_t used = 0;
@@ -455,7 +472,7 @@ void gsc_solver::solve() {
     const uint32_t S = dc.S;
     hipLaunchKernelGGL(seed_kernel, dim3(blocks_for(S, 256)), dim3(256), 0, stream, S, cand, reach, ctrl, rstep);
     GSC_HIP(hipGetLastError());
-    hipLaunchKernelGGL(reset_ctl_kernel, dim3(1), dim3(64), 0, stream, ctl, static_cast<unsigned long long>(candidate_count));
+    hipLaunchKernelGGL(reset_ctl_kernel, dim3(1), dim3(256), 0, stream, ctl, static_cast<unsigned long long>(candidate_count));
     GSC_HIP(hipGetLastError());
     GSC_HIP(hipStreamSynchronize(stream));
 
@@ -466,6 +483,9 @@ void gsc_solver::solve() {
     SatisfyArgs A{};
     A.c = dc;
     A.prefix_unreached = prefix_unreached;
+    A.prefix_valid = prefix_valid;
+    A.unreached_bits = ubits;
+    A.probe_zw = probe_zw;
     A.rec = rec;
     A.overflow = overflow;
     A.box2 = box2;
@@ -476,9 +496,8 @@ void gsc_solver::solve() {
     A.ctrl = ctrl;
     A.rstep = rstep;
     A.ctl = ctl;
-    A.chunk_states = sat_chunk;
     A.sweep_all = opts.sweep_all_pairs ? 1u : 0u;
-    const uint32_t sat_blocks = blocks_for(S, sat_chunk);
+    const uint32_t sat_blocks = dc.n_groups;  // one workgroup per 256-cell group of a plane
     const uint32_t max_it = cfg.max_iterations;
     uint32_t done = 0;
     uint32_t finished = 0;
@@ -490,9 +509,12 @@ void gsc_solver::solve() {
             hipEvent_t* ev = &events[(it - first) * kEventsPerIter];
             if (timed) GSC_HIP(hipEventRecord(ev[0], stream));
             build_prefix(valid, reach, prefix_unreached, ctl);
+            hipLaunchKernelGGL(unreached_bits_kernel, dim3(blocks_for(S, 256)), dim3(256), 0, stream, S, valid, reach,
+                               ubits, ctl);
+            GSC_HIP(hipGetLastError());
             if (timed) GSC_HIP(hipEventRecord(ev[1], stream));
             A.iter = it;
-            hipLaunchKernelGGL(satisfy_kernel, dim3(sat_blocks), dim3(256), sat_shmem, stream, A);
+            hipLaunchKernelGGL(satisfy_kernel, dim3(sat_blocks), dim3(256), 0, stream, A);
             GSC_HIP(hipGetLastError());
             if (timed) GSC_HIP(hipEventRecord(ev[2], stream));
             hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(64), 0, stream, ctl, records, it, max_it);
@@ -529,6 +551,8 @@ void gsc_solver::solve() {
                 red_ms += c;
             }
             summary.pairs_tested += r.pairs_tested;
+            summary.boxes_counted += r.boxes_counted;
+            summary.test_slots += r.test_slots;
             iter_stats.push_back(st);
             if (opts.verbose && (it <= 5 || it % 10 == 0 || it == max_it))
                 std::printf("GPU: Iteration %u/%u, newly reachable: %llu, newly certified: %llu, total certified: "

@@ -215,6 +215,8 @@ class SolveSummary(ctypes.Structure):
         ("reduction_ms", c_double),
         ("pairs_tested", c_uint64),
         ("pair_bytes_read", c_uint64),
+        ("boxes_counted", c_uint64),
+        ("test_slots", c_uint64),
         ("message", c_char * MESSAGE_LEN),
     ]
 

@@ -264,8 +264,11 @@ typedef struct gsc_solve_summary {
     double prefix_ms;            /* sum of prefix-build kernel time (HIP events) */
     double satisfaction_ms;      /* sum of pair-satisfaction kernel time (HIP events) */
     double reduction_ms;         /* sum of the per-iteration bookkeeping kernel time */
-    uint64_t pairs_tested;       /* pairs whose successor box was queried, all iterations */
+    uint64_t pairs_tested;       /* pairs whose record was read and tested, all iterations */
     uint64_t pair_bytes_read;    /* algorithmic bytes of pair records streamed, all iterations */
+    uint64_t boxes_counted;      /* tests that needed the full 16-corner prefix count */
+    uint64_t test_slots;         /* lane slots spent on tests (pairs_tested / test_slots =
+                                    lane utilisation of the satisfaction pass) */
     char message[GSC_MESSAGE_LEN];
 } gsc_solve_summary;
 
