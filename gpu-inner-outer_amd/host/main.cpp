@@ -1,0 +1,144 @@
+// main.cpp — cuda_symbolic_control_cli drop-in (src/main.cpp:125-218) on the
+// gfx950 backend: same arguments, summary and timing printout, HDF5 output
+// and exit codes (0 ok, 1 error or usage, 2 GPU run unavailable).
+//
+//   cuda_symbolic_control_cli <config.cfg> [--cpu|--gpu|--compare] [--output result.h5]
+//                             [--no-abstraction-export]
+//
+// --gpu is the backend this library provides. The reference's --cpu and
+// --compare modes run its own CPU solver, which is not part of this backend;
+// they exit 1 with a message. --no-abstraction-export (an addition) skips the
+// per-pair /abstraction datasets, whose device-to-host copy and write dominate
+// the reference's wall time at 401M pairs.
+#include <algorithm>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+
+#include "gsc/cuda_api.hpp"
+#include "gsc/result_io.hpp"
+
+namespace {
+
+void print_summary(const gsc::GpuRunReport& gpu) {
+    // main.cpp:12-39
+    std::uint64_t candidate_states = 0;
+    for (auto v : gpu.result.candidate_mask) candidate_states += (v != 0);
+    const auto& r = gpu.result;
+    std::cout << "\n[gpu]\n";
+    std::cout << "states        : " << gpu.state_grid.total_size << '\n';
+    std::cout << "inputs        : " << gpu.input_grid.total_size << '\n';
+    std::cout << "pairs         : " << gpu.pair_count << '\n';
+    std::cout << "reachable     : " << r.reachable_states << '\n';
+    std::cout << "candidate     : " << candidate_states << '\n';
+    std::cout << "candidate ctl : " << r.certified_candidate_states << '\n';
+    std::cout << "iterations    : " << r.iterations << '\n';
+    std::cout << "converged     : " << (r.converged ? "true" : "false") << '\n';
+    std::cout << "message       : " << r.message << '\n';
+    std::cout << "abstraction   : " << gpu.abstraction_ms << " ms\n";
+    std::cout << "solve         : " << r.solve_ms << " ms\n";
+    std::cout << "memory total  : " << gsc::format_bytes(gpu.budget.total_bytes) << '\n';
+}
+
+void print_iteration(const gsc::IterationStats& s) {
+    std::cout << "  Iter " << s.iteration << ": " << s.iteration_ms << " ms total"
+              << " [prefix: " << s.prefix_build_ms << " ms"
+              << ", satisfaction: " << s.satisfaction_check_ms << " ms"
+              << ", reduction: " << s.reduction_ms << " ms]"
+              << " -> " << s.newly_reachable << " new reachable, " << s.newly_certified << " new certified\n";
+}
+
+void print_gpu_detailed_timing(const gsc::GpuRunReport& gpu) {
+    // main.cpp:41-103
+    const auto& k = gpu.kernel_timings;
+    std::cout << "\n[GPU Detailed Timing]\n";
+    std::cout << "=== Abstraction Phase ===\n";
+    std::cout << "  Kernel execution      : " << k.abstraction_kernel_ms << " ms\n";
+    std::cout << "  H2D memcpy            : " << k.abstraction_memcpy_h2d_ms << " ms\n";
+    std::cout << "  Total abstraction     : " << gpu.abstraction_ms << " ms\n";
+    std::cout << "\n=== Reachability Iteration Phase ===\n";
+    std::cout << "  Prefix build (total)  : " << k.prefix_build_total_ms << " ms\n";
+    std::cout << "  Pair satisfaction     : " << k.pair_satisfaction_ms << " ms\n";
+    std::cout << "  Reduce inputs         : " << k.reduce_inputs_ms << " ms\n";
+    std::cout << "  Iteration memcpy      : " << k.iteration_memcpy_ms << " ms\n";
+    std::cout << "  Total solve           : " << gpu.solve_ms << " ms\n";
+    std::cout << "\n=== Result Copy Phase ===\n";
+    std::cout << "  D2H memcpy            : " << k.result_memcpy_d2h_ms << " ms\n";
+    std::cout << "\n=== Summary ===\n";
+    std::cout << "  Total kernel time     : " << k.total_kernel_ms << " ms\n";
+    std::cout << "  Total memcpy time     : " << k.total_memcpy_ms << " ms\n";
+    std::cout << "  Total compute time    : " << k.total_compute_ms << " ms\n";
+    std::cout << "  Total elapsed time    : " << gpu.total_ms << " ms\n";
+    const auto& m = gpu.memory_stats;
+    std::cout << "\n=== GPU Memory Usage ===\n";
+    std::cout << "  Abstraction data      : " << gsc::format_bytes(m.abstraction_bytes) << "\n";
+    std::cout << "  Prefix sum data       : " << gsc::format_bytes(m.prefix_bytes) << "\n";
+    std::cout << "  Iteration data        : " << gsc::format_bytes(m.iteration_bytes) << "\n";
+    std::cout << "  Total allocated       : " << gsc::format_bytes(m.total_allocated_bytes) << "\n";
+    std::cout << "  Peak usage            : " << gsc::format_bytes(m.peak_usage_bytes) << "\n";
+    const auto& st = gpu.iteration_stats;
+    if (st.empty()) return;
+    std::cout << "\n=== Per-Iteration Breakdown (first 5 and last 5) ===\n";
+    const int show = std::min(5, static_cast<int>(st.size()));
+    for (int i = 0; i < show; ++i) print_iteration(st[i]);
+    if (static_cast<int>(st.size()) > 2 * show) std::cout << "  ...\n";
+    for (int i = std::max(show, static_cast<int>(st.size()) - show); i < static_cast<int>(st.size()); ++i)
+        print_iteration(st[i]);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 2) {
+        std::cerr << "usage: gsc_cli <config.cfg> [--cpu|--gpu|--compare] [--output result.h5]\n";
+        return 1;
+    }
+    try {
+        const std::string config_path = argv[1];
+        std::string mode = "--cpu";
+        std::string output_path;
+        bool export_abstraction = true;
+        for (int i = 2; i < argc; ++i) {
+            const std::string arg = argv[i];
+            if (arg == "--cpu" || arg == "--gpu" || arg == "--compare") {
+                mode = arg;
+                continue;
+            }
+            if (arg == "--output") {
+                if (i + 1 >= argc) throw std::runtime_error("--output requires a file path");
+                output_path = argv[++i];
+                continue;
+            }
+            if (arg == "--no-abstraction-export") {
+                export_abstraction = false;
+                continue;
+            }
+            throw std::runtime_error("unknown argument: " + arg);
+        }
+        const auto cfg = gsc::load_case_config(config_path);
+        const std::string mode_name = mode.substr(2);
+        if (output_path.empty()) output_path = gsc::default_result_path(cfg, mode_name);
+
+        if (mode != "--gpu")
+            throw std::runtime_error("mode " + mode + " runs the reference's CPU solver, which is not part of this "
+                                     "backend; use --gpu");
+
+        const auto gpu = gsc::run_case_cuda_u32(cfg, export_abstraction);
+        if (!gpu.executed) {
+            std::cerr << "gpu run unavailable: " << gpu.message << '\n';
+            return 2;
+        }
+        print_summary(gpu);
+        print_gpu_detailed_timing(gpu);
+        gsc::write_gpu_results_hdf5(output_path, cfg, gpu);
+        std::cout << "\nresult file   : " << output_path << '\n';
+        std::cout << "  - includes abstraction data ("
+                  << (gpu.pair_count * (2 * sizeof(std::uint32_t) + sizeof(std::uint8_t)) / (1024.0 * 1024.0))
+                  << " MB)\n";
+        std::cout << "  - includes " << gpu.iteration_stats.size() << " iteration statistics\n";
+        return 0;
+    } catch (const std::exception& ex) {
+        std::cerr << "error: " << ex.what() << '\n';
+        return 1;
+    }
+}
