@@ -12,7 +12,12 @@ pairs) with a hyperbolic target band. `--workload hyperbolic_demo` runs the
 reference's published case instead.
 
 Prints one JSON line. Kernel times come from HIP events recorded by the
-library on the stream the kernels run on.
+library on the stream the kernels run on. `roofline` prices the dominant
+kernel at 9 B (the reference's per-pair record: u32 min_flat + u32 max_flat +
+u8 valid) per pair it processes; `roofline.traffic` is that kernel's measured
+HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated on a plain
+copy); `cpu_baseline` is the reference's own CPU build on a bounded slice of
+the same workload, on one host core.
 """
 import argparse
 import json
@@ -23,6 +28,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "gpu-inner-outer_amd", "python"))
 
+METRIC = "solve ms & Gpairs/s at 4.0M states×100 inputs; served solves/s + p99 latency"
 HBM_PEAK_GBS = 8000.0
 REF_PUBLISHED_GPAIRS = 23.2  # RTX 4090, hyperbolic_demo, README.md:66 / 88-98
 
@@ -58,7 +64,7 @@ def cpu_baseline(case, budget_s):
     begin = max(0, S // 2 - n_states // 2)
     probe = gsc_oracle.ref_sample(case, begin, begin + n_states, 1)
     per_sweep_ms = max(probe["satisfaction_ms"], 1e-3)
-    sweeps = int(max(1, min(50, (budget_s * 1000.0 - probe["abstraction_ms"]) / per_sweep_ms)))
+    sweeps = int(max(1, min(400, (budget_s * 1000.0 - probe["abstraction_ms"]) / per_sweep_ms)))
     r = gsc_oracle.ref_sample(case, begin, begin + n_states, sweeps)
     pairs = r["sample_pairs"]
     return {
@@ -86,6 +92,91 @@ def _cpu_model():
     return "unknown"
 
 
+# ---------------------------------------------------------------------------
+# HBM traffic of the dominant kernel, from rocprofv3 hardware counters.
+#
+# FETCH_SIZE and WRITE_SIZE (KiB per dispatch, summed over the TCC channels)
+# are collected in two separate rocprofv3 passes over a one-step child run of
+# this same script, and each is scaled by what the same counter reads for a
+# plain 1 GiB device-to-device copy (the runtime's copy kernel) measured in
+# the same way, so that the reported figure is in bytes of HBM traffic.
+# ---------------------------------------------------------------------------
+COPY_BYTES = 1 << 30
+COPY_REPEATS = 3
+
+
+def copy_calibration():
+    """Child: three 1 GiB device-to-device copies and nothing else."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(COPY_BYTES)) != 0:
+            raise SystemExit("hipMalloc failed")
+    for _ in range(COPY_REPEATS):
+        if hip.hipMemcpy(dst, src, ctypes.c_size_t(COPY_BYTES), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise SystemExit("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copied_bytes": COPY_BYTES, "repeats": COPY_REPEATS}))
+
+
+def _pmc_pass(counter, child_args, outdir, timeout_s=150):
+    import csv
+    import subprocess
+
+    d = os.path.join(outdir, counter + "_" + ("copy" if "--copy-calibration" in child_args else "run"))
+    cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
+           sys.executable, os.path.abspath(__file__)] + child_args
+    env = dict(os.environ, TMPDIR="/tmp")
+    print(f"[bench] pmc pass {counter}: {' '.join(child_args)}", file=sys.stderr, flush=True)
+    p = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True, timeout=timeout_s)
+    if p.returncode != 0:
+        raise RuntimeError(f"rocprofv3 --pmc {counter} exited {p.returncode}: {p.stderr[-400:]}")
+    child = None
+    for line in p.stdout.splitlines():
+        if line.startswith("{"):
+            child = json.loads(line)
+    per_kernel = {}
+    with open(os.path.join(d, "run_counter_collection.csv")) as f:
+        for row in csv.DictReader(f):
+            if row["Counter_Name"] != counter:
+                continue
+            k = row["Kernel_Name"].split("(")[0].split("::")[-1]
+            n, s = per_kernel.get(k, (0, 0.0))
+            per_kernel[k] = (n + 1, s + float(row["Counter_Value"]))
+    return child, per_kernel
+
+
+def measure_traffic(workload_name, kernel, sweep_all, outdir):
+    """HBM bytes per launch of `kernel`: (FETCH_SIZE x f_fetch + WRITE_SIZE x f_write) x 1024,
+    f = known copy bytes / counter reading of the copy kernel, averaged over the work launches."""
+    run_args = ["--workload", workload_name, "--steps", "1", "--warmup", "0", "--no-cpu-baseline", "--traffic", "off"]
+    if sweep_all:
+        run_args.append("--sweep-all-pairs")
+    out = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        _, copy_k = _pmc_pass(counter, ["--copy-calibration"], outdir)
+        n_copy, kib_copy = copy_k.get("__amd_rocclr_copyBuffer", (0, 0.0))
+        if kib_copy <= 0:
+            raise RuntimeError(f"copy kernel not seen in the {counter} calibration pass")
+        factor = (COPY_BYTES / 1024.0) * COPY_REPEATS / kib_copy
+        child, run_k = _pmc_pass(counter, run_args, outdir)
+        n, kib = run_k.get(kernel, (0, 0.0))
+        if n == 0 or child is None:
+            raise RuntimeError(f"{kernel} not seen in the {counter} pass")
+        # The satisfaction kernel is launched in batches of 8; launches after the
+        # fixpoint has stopped return at once. Average over the iterations run.
+        units = child["config"]["iterations"] if kernel == "satisfy_kernel" else n
+        out[counter] = {"kib_total": kib, "dispatches": n, "launches_counted": units,
+                        "copy_kernel_dispatches": n_copy, "copy_kib_per_gib": kib_copy / COPY_REPEATS,
+                        "factor": factor, "bytes_per_launch": kib * 1024.0 * factor / units}
+    out["bytes_per_launch"] = out["FETCH_SIZE"]["bytes_per_launch"] + out["WRITE_SIZE"]["bytes_per_launch"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,7 +187,14 @@ def main():
                     help="test every pair every iteration (the reference's work schedule)")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of reference CPU work for the baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic", choices=("live", "off"), default="live",
+                    help="measure the dominant kernel's HBM bytes with two rocprofv3 --pmc passes")
+    ap.add_argument("--pmc-dir", default=os.path.join(ROOT, "gpurun_out", "bench_pmc"))
+    ap.add_argument("--copy-calibration", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
         sys.exit(2)
@@ -131,24 +229,28 @@ def main():
     summary = s
     K = args.steps
     gpairs = P * iters / (sat_ms / 1000.0) / 1e9
-    # Dominant kernel per step decides the roofline line.
+    # Dominant kernel per step decides the roofline line. Algorithmic bytes are
+    # the reference pair record (u32 min_flat + u32 max_flat + u8 valid = 9 B)
+    # per pair the launch processes: every pair in the reference's schedule,
+    # the pairs of still-unsettled states in the default one.
     sat_launch_ms = sat_ms / iters
     S = solver.state_count
-    sat_bytes_per_launch = (4.0 * tested + 7.0 * S * iters) / iters
     abs_launch_ms = abs_ms / K
     if sat_ms >= abs_ms:
-        roof = {"kernel": "satisfy_kernel", "bound": "hbm", "achieved": sat_bytes_per_launch / (sat_launch_ms / 1000.0) / 1e9,
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None,
-                "launch_ms": sat_launch_ms, "algorithmic_bytes_per_launch": sat_bytes_per_launch}
+        units = tested / iters
+        roof = {"kernel": "satisfy_kernel", "bound": "hbm", "launch_ms": sat_launch_ms,
+                "algorithmic_bytes_per_launch": 9.0 * units, "pairs_processed_per_launch": units,
+                "effective_GBs_all_pairs": 9.0 * P / (sat_launch_ms / 1000.0) / 1e9}
     else:
-        abs_bytes = 4.0 * P
-        roof = {"kernel": "abstraction_kernel", "bound": "hbm", "achieved": abs_bytes / (abs_launch_ms / 1000.0) / 1e9,
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None,
-                "launch_ms": abs_launch_ms, "algorithmic_bytes_per_launch": abs_bytes}
+        roof = {"kernel": "abstraction_kernel", "bound": "hbm", "launch_ms": abs_launch_ms,
+                "algorithmic_bytes_per_launch": 9.0 * P, "pairs_processed_per_launch": P}
+    roof["achieved"] = roof["algorithmic_bytes_per_launch"] / (roof["launch_ms"] / 1000.0) / 1e9
+    roof.update({"peak": HBM_PEAK_GBS, "unit": "GB/s"})
     roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
 
     line = {
-        "metric": "pair-satisfaction Gpairs/s (pairs x iterations / satisfaction time) and solve ms at 4.0M states x 100 inputs",
+        "metric": METRIC,
         "value": gpairs,
         "unit": "Gpairs/s",
         "n_gpus": 1,
@@ -185,6 +287,13 @@ def main():
         "cpu_baseline": None,
     }
     solver.close()
+    if args.traffic == "live":
+        try:
+            t = measure_traffic(args.workload, roof["kernel"], args.sweep_all_pairs, os.path.abspath(args.pmc_dir))
+            roof["traffic"] = t["bytes_per_launch"]
+            roof["traffic_detail"] = t
+        except Exception as e:  # counters unavailable: traffic stays null, the reason is kept
+            roof["traffic_error"] = str(e)[:300]
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(case, args.cpu_budget)
     print(json.dumps(line))
