@@ -4,7 +4,7 @@
 // Every function here that turns FP64 coordinates into integer cells must give
 // the same bits on the host and on the device. Both sides are compiled with
 // -ffp-contract=off (no FMA contraction), keep the reference's expression
-// order (grid.hpp:72, 106, 211-281; model.hpp:34-45, 59-75, 107-111) and use
+// order (grid.hpp:42, 76, 181-251; model.hpp:34-45, 59-75, 107-111) and use
 // only IEEE-exact operations (+, -, *, correctly rounded /, floor, ceil).
 #pragma once
 
@@ -18,7 +18,7 @@ namespace gsc_amd {
 
 #define GSC_HD __host__ __device__
 
-constexpr double kEps = 1e-9;  // grid.hpp:216
+constexpr double kEps = 1e-9;  // grid.hpp:186
 
 // ---- pair record (4 bytes per (state, input) pair) ------------------------
 // The reference stores min_flat u32 + max_flat u32 + valid u8 per pair
@@ -33,7 +33,7 @@ constexpr double kEps = 1e-9;  // grid.hpp:216
 //   bits 25-29  y extent
 //   bit  30     dead: the box holds no valid cell, so the pair can never be
 //               satisfied (the reference's `valid_count == 0` test,
-//               reachability.hpp:387-394 / kernels.cu:327-332). The valid set
+//               reachability.hpp:78-85 / kernels.cu:327-332). The valid set
 //               never changes, so the answer is memoised: the abstraction
 //               writes 0 ("not known dead") and the satisfaction pass sets the
 //               bit the first time it counts zero valid cells in the box.
@@ -72,12 +72,12 @@ struct OverflowEntry {
 // ---- case parameters passed by value to every kernel -----------------------
 struct DevCase {
     double lb[4], ub[4], eta[4];
-    double lb_minus_eps[4];  // lb[d] - kEps   (grid.hpp:223)
+    double lb_minus_eps[4];  // lb[d] - kEps   (grid.hpp:193)
     double ub_plus_eps[4];   // ub[d] + kEps
-    double wrap_span_minus_eps;  // (ub[w] - lb[w]) - kEps (grid.hpp:228-233)
+    double wrap_span_minus_eps;  // (ub[w] - lb[w]) - kEps (grid.hpp:198-207)
     uint32_t shape[4];
-    uint32_t stride[4];   // state strides, x fastest (grid.hpp:156-159)
-    uint32_t pstride[4];  // padded prefix strides (grid.hpp:283-299)
+    uint32_t stride[4];   // state strides, x fastest (grid.hpp:126-129)
+    uint32_t pstride[4];  // padded prefix strides (grid.hpp:253-269)
     uint32_t S;           // states
     uint32_t I;           // inputs
     uint32_t I_pad;       // inputs rounded up to a whole number of satisfaction blocks;
@@ -101,7 +101,7 @@ struct DevCase {
 };
 
 GSC_HD inline double cell_center(double lb, uint32_t idx, double eta) {
-    // grid.hpp:72 / :106
+    // grid.hpp:42 / :76
     return lb + (static_cast<double>(idx) + 0.5) * eta;
 }
 
@@ -137,7 +137,7 @@ GSC_HD inline double wrap_angle(double x) {
     return x;
 }
 
-// Grid4D::wrap_value, grid.hpp:76-87.
+// Grid4D::wrap_value, grid.hpp:46-57.
 GSC_HD inline double wrap_value(double low, double high, double x) {
     const double span = high - low;
     while (x < low) x += span;
@@ -145,7 +145,7 @@ GSC_HD inline double wrap_value(double low, double high, double x) {
     return x;
 }
 
-// One dimension of continuous_box_to_index_box (grid.hpp:211-281).
+// One dimension of continuous_box_to_index_box (grid.hpp:181-251).
 // Returns false when a non-wrap dimension leaves [lb - eps, ub + eps].
 GSC_HD inline bool dim_to_cells(const DevCase& c, int d, double mn, double mx, uint32_t* lo, uint32_t* hi) {
     double a = mn, b = mx;

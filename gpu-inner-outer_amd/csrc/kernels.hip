@@ -10,7 +10,7 @@
 //   satisfy_kernel       pair satisfaction fused with the first-input reduction
 //                        (kernels.cu:310-393, reachability.hpp:59-109)
 //   finish_kernel        per-iteration totals and the stop test
-//                        (kernels.cu:722-766, reachability.hpp:420-452)
+//                        (kernels.cu:722-766, reachability.hpp:111-152)
 //   export_kernel        reference abstraction triplet from the pair records
 //
 // No MFMA anywhere: nothing here is a dense contraction. The satisfaction pass
@@ -57,7 +57,7 @@ __device__ inline bool loop_done(const IterCtl* ctl) {
 }
 
 __device__ inline void unflatten(const DevCase& c, uint32_t s, uint32_t idx[4]) {
-    // grid.hpp:60-66 with x fastest.
+    // grid.hpp:30-36 with x fastest.
     uint32_t t = s / c.shape[0];
     idx[0] = s - t * c.shape[0];
     uint32_t t2 = t / c.shape[1];
@@ -282,11 +282,11 @@ __global__ __launch_bounds__(256) void probe_zw_kernel(DevCase c, const uint2* _
 }
 
 // Abstraction. Per pair, the arithmetic is the reference's in the reference's order:
-//   centre           grid.hpp:72
+//   centre           grid.hpp:42
 //   nominal x, y     model.hpp:35-42: x += dt*(v_k*cos θ_k), y += dt*(v_k*sin θ_k),
 //                    the increments read from increments_kernel's table
 //   radius           model.hpp:71-73, 84-89, 107-111 (per v cell, host table)
-//   cells            grid.hpp:211-281 (dims 0,1 here; dims 2,3 from box2/box3)
+//   cells            grid.hpp:181-251 (dims 0,1 here; dims 2,3 from box2/box3)
 //   constraint scan  abstraction.hpp:110-159 as a 2D prefix query over x,y fails
 // The `dead` bit of compact records is left 0 here and settled lazily by the
 // satisfaction pass (see satisfy_kernel); escaped pairs get it eagerly.
@@ -624,7 +624,7 @@ __global__ __launch_bounds__(256) void satisfy_kernel(SatisfyArgs A) {
             }
             if (!A.sweep_all && __ballot(pending) == 0) break;
         }
-        // 3. update the state (reachability.hpp:401-415, kernels.cu:376-392).
+        // 3. update the state (reachability.hpp:93-105, kernels.cu:376-392).
         if ((flags & kUse) && found != GSC_INVALID_INPUT) {
             if (!(flags & kReached)) {
                 A.reach[s] = 1;

@@ -2,7 +2,7 @@
 //
 // A gsc_solver owns one case on one device: its tables, masks, pair records,
 // prefix tables and iteration state, all resident in HBM. The fixpoint loop
-// (kernels.cu:654-767 / reachability.hpp:362-452) is launched in batches of
+// (kernels.cu:654-767 / reachability.hpp:30-155) is launched in batches of
 // iterations with no host synchronisation inside a batch: every kernel reads a
 // device-side `done` flag and returns at once after the stop test fired, so a
 // batch past the last iteration costs a few empty launches and the stop
@@ -62,7 +62,7 @@ const char* done_message(uint32_t done) {
     }
 }
 
-// build_state_grid / build_input_grid / build_prefix_layout (grid.hpp:133-198, 283-299),
+// build_state_grid / build_input_grid / build_prefix_layout (grid.hpp:103-168, 253-269),
 // with the reference's error texts.
 gsc_case_shape make_shape(const gsc_case_desc& c) {
     gsc_case_shape sh;

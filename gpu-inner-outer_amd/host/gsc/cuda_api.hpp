@@ -1,5 +1,5 @@
 // gsc/cuda_api.hpp — the GPU backend boundary with the reference's names
-// (include/gsc/cuda_api.hpp:11-88, reachability.hpp:15-28, grid.hpp:42-109,
+// (include/gsc/cuda_api.hpp:11-88, reachability.hpp:15-28, grid.hpp:12-79,
 // abstraction.hpp:16-30), implemented over the C ABI of libgsc_amd.so
 // (gsc_run_case_u32). A caller of the reference's run_case_cuda_u32 compiles
 // against this header unchanged.
@@ -32,7 +32,7 @@ struct InputGrid2D {
     std::uint64_t strides[kInputDim]{};
     std::uint64_t total_size = 0;
 
-    // Cell centre of input `flat` (grid.hpp:98-108).
+    // Cell centre of input `flat` (grid.hpp:68-78).
     void center(InputIndex flat, double out[kInputDim]) const {
         std::uint64_t v = flat;
         std::uint32_t idx[kInputDim]{};

@@ -13,7 +13,7 @@ The generators build the benchmark workloads:
   unicycle_128x128x64x16 16.8M cells x 100 inputs
   serving requests       log-uniform 20k-4M cells
 Every shape is checked against the reference's ceil((ub-lb)/eta) rule
-(grid.hpp:153), which can add a cell when the quotient rounds up.
+(grid.hpp:121, 157), which can add a cell when the quotient rounds up.
 """
 from __future__ import annotations
 
@@ -176,7 +176,7 @@ class Case:
         return "\n".join(l for l in self.to_cfg().splitlines() if not l.startswith(("target =", "target_band ="))) + "\n"
 
     def shape(self) -> Tuple[Tuple[int, ...], Tuple[int, int]]:
-        """Grid shapes by the reference rule ceil((ub - lb) / eta) (grid.hpp:153, 187)."""
+        """Grid shapes by the reference rule ceil((ub - lb) / eta) (grid.hpp:121, 157)."""
         s = tuple(int(math.ceil((self.state_ub[d] - self.state_lb[d]) / self.state_eta[d])) for d in range(4))
         i = tuple(int(math.ceil((self.input_ub[d] - self.input_lb[d]) / self.input_eta[d])) for d in range(2))
         return s, i

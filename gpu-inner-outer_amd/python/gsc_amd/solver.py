@@ -1,7 +1,7 @@
 """Python-side mirror of the backend entry points, over the C ABI.
 
 `run_case_u32(case)` is the counterpart of the reference's
-`gsc::run_case_cuda_u32(const CaseConfig&)` (include/gsc/cuda_api.hpp:170):
+`gsc::run_case_cuda_u32(const CaseConfig&)` (include/gsc/cuda_api.hpp:88):
 it returns the masks, controller and reach-step tables, per-iteration
 statistics and timings, and — like the reference — raises for preparation
 errors (kernels.cu:505 lets them escape) while device errors are reported in

@@ -2,8 +2,8 @@
  * gsc_amd.h — C ABI of the MI355X (gfx950) grid symbolic-control backend.
  *
  * The reference exposes its GPU backend as two C++ symbols,
- *   bool gsc::cuda_backend_compiled();                       (include/gsc/cuda_api.hpp:169)
- *   GpuRunReport gsc::run_case_cuda_u32(const CaseConfig&);  (include/gsc/cuda_api.hpp:170)
+ *   bool gsc::cuda_backend_compiled();                       (include/gsc/cuda_api.hpp:87)
+ *   GpuRunReport gsc::run_case_cuda_u32(const CaseConfig&);  (include/gsc/cuda_api.hpp:88)
  * whose arguments carry std::string / std::vector / std::function members
  * (include/gsc/config.hpp:41-67). This header re-states that boundary as plain
  * C: POD descriptors, caller-owned buffers, sizes and a status code, so that
@@ -37,7 +37,7 @@ typedef enum gsc_status {
     GSC_ERR_INVALID_ARGUMENT = 1, /* null pointer, bad size or buffer too small */
     GSC_ERR_CONFIG = 2,           /* preparation failed; the reference throws these out of
                                      run_case_cuda_u32 (kernels.cu:505, abstraction.hpp:261-265,
-                                     grid.hpp:146-167) */
+                                     grid.hpp:114-137, 150-166) */
     GSC_ERR_NO_DEVICE = 3,        /* kernels.cu:496-501 "no CUDA device found" */
     GSC_ERR_DEVICE = 4,           /* HIP runtime error (kernels.cu:21-27, caught at :856-858) */
     GSC_ERR_OUT_OF_MEMORY = 5,    /* device allocation failed or HBM budget exceeded */
@@ -95,7 +95,7 @@ typedef struct gsc_case_desc {
     double target_band_w;
 } gsc_case_desc;
 
-/* Grid4D / InputGrid2D / PrefixLayout4D sizes (grid.hpp:42-131, 133-198, 283-299). */
+/* Grid4D / InputGrid2D / PrefixLayout4D sizes (grid.hpp:12-101, 103-168, 253-269). */
 typedef struct gsc_case_shape {
     uint32_t state_shape[GSC_STATE_DIM];
     uint64_t state_strides[GSC_STATE_DIM];
@@ -211,7 +211,7 @@ typedef struct gsc_result_buffers {
     uint32_t iteration_stats_capacity;
 } gsc_result_buffers;
 
-/* Replaces gsc::cuda_backend_compiled (cuda_api.hpp:169, kernels.cu:488-490). Returns 1. */
+/* Replaces gsc::cuda_backend_compiled (cuda_api.hpp:87, kernels.cu:488-490). Returns 1. */
 int gsc_backend_compiled(void);
 
 /* Number of visible HIP devices (kernels.cu:496-497); 0 when none or on error. */
@@ -219,13 +219,13 @@ int gsc_device_count(void);
 
 /*
  * Grid shapes of a case (build_state_grid / build_input_grid / build_prefix_layout,
- * grid.hpp:133-198, 283-299). Fails with GSC_ERR_CONFIG and the reference's
+ * grid.hpp:103-168, 253-269). Fails with GSC_ERR_CONFIG and the reference's
  * message for non-positive eta, ub <= lb or a state count above 2^32-1.
  */
 int gsc_case_shape_of(const gsc_case_desc* cfg, gsc_case_shape* out, char* err, size_t err_len);
 
 /*
- * Replaces gsc::run_case_cuda_u32 (cuda_api.hpp:170, kernels.cu:492-875): build the
+ * Replaces gsc::run_case_cuda_u32 (cuda_api.hpp:88, kernels.cu:492-875): build the
  * masks, the pair abstraction, then iterate the backward fixpoint to convergence,
  * stall or max_iterations, and copy the requested results to `out`.
  * Returns GSC_OK with report->executed = 1 on success. Preparation failures the

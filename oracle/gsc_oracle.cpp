@@ -5,9 +5,9 @@
 //
 // It follows the reference CPU path line by line in FP64 with no FMA
 // contraction (build with -O2 -ffp-contract=off, never -march=native):
-//   grids          include/gsc/grid.hpp:133-198, 283-299
-//   cell centres   include/gsc/grid.hpp:68-74, 98-108
-//   box rounding   include/gsc/grid.hpp:211-281
+//   grids          include/gsc/grid.hpp:103-168, 253-269
+//   cell centres   include/gsc/grid.hpp:38-44, 68-78
+//   box rounding   include/gsc/grid.hpp:181-251
 //   model          include/gsc/model.hpp:14-112
 //   masks          include/gsc/abstraction.hpp:65-106, config.cpp:64-86, 111-127
 //   abstraction    include/gsc/abstraction.hpp:110-209
@@ -46,14 +46,14 @@ struct Grid {
     int wrap_dim;
 
     uint32_t flatten(const uint32_t idx[kD]) const {
-        // grid.hpp:52-58: u64 arithmetic, cast to the u32 state index.
+        // grid.hpp:22-28: u64 arithmetic, cast to the u32 state index.
         return static_cast<uint32_t>(static_cast<uint64_t>(idx[0]) * strides[0] +
                                      static_cast<uint64_t>(idx[1]) * strides[1] +
                                      static_cast<uint64_t>(idx[2]) * strides[2] +
                                      static_cast<uint64_t>(idx[3]) * strides[3]);
     }
     void unflatten(uint32_t flat, uint32_t idx[kD]) const {
-        // grid.hpp:60-66
+        // grid.hpp:30-36
         uint64_t v = flat;
         for (int d = kD - 1; d >= 0; --d) {
             idx[d] = static_cast<uint32_t>(v / strides[d]);
@@ -61,13 +61,13 @@ struct Grid {
         }
     }
     void center(uint32_t flat, double out[kD]) const {
-        // grid.hpp:68-74
+        // grid.hpp:38-44
         uint32_t idx[kD];
         unflatten(flat, idx);
         for (int d = 0; d < kD; ++d) out[d] = lb[d] + (static_cast<double>(idx[d]) + 0.5) * eta[d];
     }
     double wrap_value(double x) const {
-        // grid.hpp:76-87
+        // grid.hpp:46-57
         const double low = lb[wrap_dim];
         const double high = ub[wrap_dim];
         const double span = high - low;
@@ -83,7 +83,7 @@ struct Inputs {
     uint64_t strides[2];
     uint64_t total;
     void center(uint32_t flat, double out[2]) const {
-        // grid.hpp:98-108
+        // grid.hpp:68-78
         uint64_t v = flat;
         uint32_t idx[2] = {0, 0};
         for (int d = 1; d >= 0; --d) {
@@ -94,7 +94,7 @@ struct Inputs {
     }
 };
 
-struct Layout {  // PrefixLayout4D, grid.hpp:117-131, 283-299
+struct Layout {  // PrefixLayout4D, grid.hpp:87-101, 253-269
     uint32_t pshape[kD];
     uint64_t strides[kD];
     uint64_t total;
@@ -109,7 +109,7 @@ struct IBox {
 };
 
 Grid make_grid(const gsc_case_desc& c) {
-    // grid.hpp:133-169
+    // grid.hpp:103-139
     Grid g{};
     g.wrap_dim = c.wrap_dim;
     for (int d = 0; d < kD; ++d) {
@@ -131,7 +131,7 @@ Grid make_grid(const gsc_case_desc& c) {
 }
 
 Inputs make_inputs(const gsc_case_desc& c) {
-    // grid.hpp:171-198
+    // grid.hpp:141-168
     Inputs in{};
     for (int d = 0; d < 2; ++d) {
         in.lb[d] = c.input_lb[d];
@@ -253,7 +253,7 @@ int64_t clamp_cell(int64_t v, uint32_t n) {
 }
 
 bool to_index_box(const Grid& g, const double mn[kD], const double mx[kD], IBox* out) {
-    // grid.hpp:211-281. Note the two separate clamps of lower and upper.
+    // grid.hpp:181-251. Note the two separate clamps of lower and upper.
     constexpr double kEps = 1e-9;
     for (int d = 0; d < kD; ++d) {
         if (d == g.wrap_dim) continue;

@@ -11,7 +11,7 @@
 //       times the reference's per-pair work on states [begin, end):
 //       successor_box + continuous_box_to_index_box + box_satisfies_constraint
 //       (abstraction.hpp:172-205), and `iters` sweeps of the pair-satisfaction
-//       test (two query_box_count calls per valid pair, reachability.hpp:381-401)
+//       test (two query_box_count calls per valid pair, reachability.hpp:78-93)
 //       against prefix tables of the valid mask and the candidate (seed) mask.
 //
 // The extension options rebuild the candidate mask the same way the

@@ -3,7 +3,7 @@
 - libgsc_amd.so loads and exports every entry point declared in include/gsc_amd.h;
 - the .cfg reader keeps the reference's keys, defaults, traps and error texts
   (src/config.cpp:243-347), checked against the reference build where present;
-- grid shapes and their errors (grid.hpp:133-198).
+- grid shapes and their errors (grid.hpp:103-168).
 """
 import ctypes
 import os

@@ -169,7 +169,7 @@ def test_hdf5_writer_layout_and_values(tmp_path):
     assert '"fixture case"' in _h5_text(str(out), "/meta/case_name")
     assert '"all candidate states obtained a controller"' in _h5_text(str(out), "/summary/gpu/message")
     assert _h5_text(str(out), "/meta/run_count").split(":")[-1].strip() == "1"
-    # input centres: lb + (idx + 0.5) * eta, x fastest (grid.hpp:98-108)
+    # input centres: lb + (idx + 0.5) * eta, x fastest (grid.hpp:68-78)
     c = _h5_array(str(out), "/input/centers", np.float64, t).reshape(6, 2)
     lb, eta = (-0.5, -1.0), (0.5, 1.0)
     want = [[lb[0] + (k % 2 + 0.5) * eta[0], lb[1] + (k // 2 + 0.5) * eta[1]] for k in range(6)]

@@ -170,7 +170,7 @@ def test_box_count_matches_brute_force(oracle):
 
 
 def test_shape_rule_and_ceil_trap(oracle):
-    # grid.hpp:153: ceil((ub - lb) / eta); (1.1 - (-0.1)) / 0.1 = 12.000000000000002 -> 13 cells.
+    # grid.hpp:121: ceil((ub - lb) / eta); (1.1 - (-0.1)) / 0.1 = 12.000000000000002 -> 13 cells.
     sh = oracle.shape(cases.hyperbolic_demo())
     assert sh["state_shape"] == (70, 70, 63, 13)
     assert sh["pair_count"] == 401_310_000
