@@ -123,7 +123,14 @@ __global__ __launch_bounds__(256) void seed_kernel(uint32_t S, const uint8_t* __
     rstep[s] = c ? 0u : GSC_UNREACHABLE_STEP;
 }
 
-__global__ void reset_ctl_kernel(IterCtl* ctl, unsigned long long candidate_count) {
+// p[0..n) = value (a kernel rather than hipMemsetAsync, so callers may capture it).
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t n, uint32_t value) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = value;
+}
+
+// Control block back to "iteration 0, not done", counters zeroed (by a kernel,
+// never hipMemset, so the reset can sit in a captured graph).
+__device__ __forceinline__ void reset_ctl_body(IterCtl* ctl, unsigned long long candidate_count) {
     const uint32_t t = threadIdx.x;
     if (t == 0) {
         ctl->done = 0;
@@ -142,15 +149,19 @@ __global__ void reset_ctl_kernel(IterCtl* ctl, unsigned long long candidate_coun
     }
 }
 
+__global__ void reset_ctl_kernel(IterCtl* ctl, unsigned long long candidate_count) {
+    reset_ctl_body(ctl, candidate_count);
+}
+
 // ---------------------------------------------------------------------------
 // Prefix table of (a & !b) over the padded layout. Padding entries (index 0 on
 // any axis) are zero from allocation and never written.
-// Pass 1: one wave per interior (y,z,w) line; wave scan along x.
-__global__ __launch_bounds__(256) void prefix_rows_kernel(DevCase c, const uint8_t* __restrict__ a,
-                                                           const uint8_t* __restrict__ b, uint32_t* __restrict__ P,
-                                                           const IterCtl* ctl) {
-    if (ctl && loop_done(ctl)) return;
-    const uint32_t line = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+// Pass 1: one wave per interior (y,z,w) line; wave scan along x. `block` covers
+// 4 lines.
+__device__ __forceinline__ void prefix_rows_body(const DevCase& c, const uint8_t* __restrict__ a,
+                                                 const uint8_t* __restrict__ b, uint32_t* __restrict__ P,
+                                                 uint32_t block) {
+    const uint32_t line = block * 4u + (threadIdx.x >> 6);
     const uint32_t lines = c.shape[1] * c.shape[2] * c.shape[3];
     if (line >= lines) return;
     const uint32_t y = line % c.shape[1];
@@ -170,24 +181,38 @@ __global__ __launch_bounds__(256) void prefix_rows_kernel(DevCase c, const uint8
     }
 }
 
-// Bitmask of valid, unreached states (bit s of word s / 64), for the probe in
-// satisfy_kernel. Thread per state; lane 0 of each wave stores the wave's ballot.
-__global__ __launch_bounds__(256) void unreached_bits_kernel(uint32_t S, const uint8_t* __restrict__ valid,
-                                                              const uint8_t* __restrict__ reach,
-                                                              unsigned long long* __restrict__ bits, const IterCtl* ctl) {
+__global__ __launch_bounds__(256) void prefix_rows_kernel(DevCase c, const uint8_t* __restrict__ a,
+                                                           const uint8_t* __restrict__ b, uint32_t* __restrict__ P,
+                                                           const IterCtl* ctl) {
     if (ctl && loop_done(ctl)) return;
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    prefix_rows_body(c, a, b, P, blockIdx.x);
+}
+
+// Bitmask of valid, unreached states (bit s of word s / 64), for the probe in
+// satisfy_kernel. Thread per state (256 per block); lane 0 of each wave stores
+// the wave's ballot.
+__device__ __forceinline__ void unreached_bits_body(uint32_t S, const uint8_t* __restrict__ valid,
+                                                    const uint8_t* __restrict__ reach,
+                                                    unsigned long long* __restrict__ bits, uint32_t block) {
+    const uint32_t s = block * 256u + threadIdx.x;
     const bool u = s < S && valid[s] != 0 && reach[s] == 0;
     const uint64_t b = __ballot(u);
     if (lane_id() == 0 && s < S) bits[s >> 6] = b;
 }
 
-// Pass 2..4: running sum along `axis` (1, 2 or 3); one thread per line, lines
-// enumerated with x fastest so neighbouring threads touch neighbouring words.
-__global__ __launch_bounds__(256) void prefix_axis_kernel(DevCase c, int axis, uint32_t* __restrict__ P,
-                                                           const IterCtl* ctl) {
+__global__ __launch_bounds__(256) void unreached_bits_kernel(uint32_t S, const uint8_t* __restrict__ valid,
+                                                              const uint8_t* __restrict__ reach,
+                                                              unsigned long long* __restrict__ bits, const IterCtl* ctl) {
     if (ctl && loop_done(ctl)) return;
-    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    unreached_bits_body(S, valid, reach, bits, blockIdx.x);
+}
+
+// Pass 2..4: running sum along `axis` (1, 2 or 3); one thread per line (256 per
+// block), lines enumerated with x fastest so neighbouring threads touch
+// neighbouring words.
+__device__ __forceinline__ void prefix_axis_body(const DevCase& c, int axis, uint32_t* __restrict__ P,
+                                                 uint32_t block) {
+    const uint32_t tid = block * 256u + threadIdx.x;
     // The two other non-x axes.
     const int o1 = axis == 1 ? 2 : 1;
     const int o2 = axis == 3 ? 2 : 3;
@@ -220,6 +245,12 @@ __global__ __launch_bounds__(256) void prefix_axis_kernel(DevCase c, int axis, u
         P[off] = acc;
         off += step;
     }
+}
+
+__global__ __launch_bounds__(256) void prefix_axis_kernel(DevCase c, int axis, uint32_t* __restrict__ P,
+                                                           const IterCtl* ctl) {
+    if (ctl && loop_done(ctl)) return;
+    prefix_axis_body(c, axis, P, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -484,200 +515,11 @@ __device__ inline bool count_test(const SatisfyArgs& A, uint32_t r, uint32_t ix,
     return true;
 }
 
-// Pair satisfaction fused with the first-input reduction.
-//
-// A workgroup owns a group of 256 consecutive cells of one (θ, v) plane. It
-// first compacts the states that need testing into LDS, so each wave runs 64
-// of them (no lanes spent on settled or invalid states). Each wave then walks
-// the inputs in increasing order in blocks of kInputBlock: the block's record
-// loads are issued together, then a bitmask probe per pair reads the whole
-// x-row of the successor box at its middle y (θ and v at their lower ends) —
-// any valid unreached cell there proves the pair unsatisfied — and only the
-// pairs the probe cannot rule out get the full prefix count. A lane stops at
-// its first satisfied input and the wave when no lane is left (the reference
-// breaks at the first satisfied input, reachability.hpp:106 /
-// kernels.cu:369-374). With sweep_all every pair of every state is tested, the
-// reference kernel's schedule (kernels.cu:668).
-// States are updated in place: the prefix snapshot was built before this
-// launch and only this workgroup touches these states (Jacobi, as kernels.cu:742).
-__global__ __launch_bounds__(256) void satisfy_kernel(SatisfyArgs A) {
-    if (loop_done(A.ctl)) return;
-    const DevCase& c = A.c;
-    constexpr uint32_t kUse = 1, kCand = 2, kReached = 4, kCertified = 8;
-    __shared__ uint32_t s_q[256];
-    __shared__ uint8_t s_flags[256];
-    __shared__ uint32_t s_wave_n[4];
-    __shared__ uint32_t s_counts[4][5];
-    const uint32_t lane = lane_id();
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t plane = blockIdx.x / c.groups_per_plane;
-    const uint32_t g0 = (blockIdx.x - plane * c.groups_per_plane) * 256u;
-    const uint32_t PQ = c.PQ;
-
-    // 1. flags of the group's cells, compacted in LDS (cell order kept).
-    {
-        const uint32_t q = g0 + threadIdx.x;
-        const uint32_t s = plane * PQ + q;
-        uint32_t f = 0;
-        bool eval = false;
-        if (q < PQ) {
-            if (A.valid[s]) {
-                const bool is_cand = A.cand[s] != 0;
-                const bool reached = A.reach[s] != 0;
-                const bool certified = A.ctrl[s] != GSC_INVALID_INPUT;
-                const bool use = !((!is_cand && reached) || (is_cand && certified));
-                f = (use ? kUse : 0) | (is_cand ? kCand : 0) | (reached ? kReached : 0) | (certified ? kCertified : 0);
-            }
-            eval = A.sweep_all ? true : (f & kUse) != 0;
-        }
-        const uint64_t m = __ballot(eval);
-        if (lane == 0) s_wave_n[wave] = static_cast<uint32_t>(__popcll(m));
-        __syncthreads();
-        uint32_t base = 0;
-        for (uint32_t w = 0; w < wave; ++w) base += s_wave_n[w];
-        if (eval) {
-            const uint32_t pos = base + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
-            s_q[pos] = q;
-            s_flags[pos] = static_cast<uint8_t>(f);
-        }
-        __syncthreads();
-    }
-    const uint32_t n_eval = s_wave_n[0] + s_wave_n[1] + s_wave_n[2] + s_wave_n[3];
-
-    // 2. each wave tests 64 of the compacted states.
-    uint32_t nr = 0, nc = 0, tested = 0, counted = 0, slots = 0;
-    const uint32_t e = wave * 64u + lane;
-    const bool has = e < n_eval;
-    if (wave * 64u < n_eval) {
-        const uint32_t q = has ? s_q[e] : g0;
-        const uint32_t flags = has ? s_flags[e] : 0u;
-        const uint32_t s = plane * PQ + q;
-        const uint32_t iy = q / c.shape[0];
-        const uint32_t ix = q - iy * c.shape[0];
-        const uint32_t lane_cell = ix + iy * c.stride[1];
-        const uint32_t st1 = c.stride[1];
-        // Everything indexed by (plane, input) is wave-uniform: the record slice
-        // base (32-bit lane offsets from it) and the probe's θ/v cell part.
-        uint32_t* const rec_plane = A.rec + static_cast<uint64_t>(plane) * c.I_pad * PQ;
-        const ConstPtr<uint32_t> probe_zw = to_const(A.probe_zw + static_cast<size_t>(plane) * c.I_pad);
-        const uint32_t* const ubits = reinterpret_cast<const uint32_t*>(A.unreached_bits);
-        uint32_t found = GSC_INVALID_INPUT;
-        bool pending = has;
-        for (uint32_t i0 = 0; i0 < c.I_pad; i0 += kInputBlock) {
-            const uint32_t nb = min(kInputBlock, c.I - i0);  // real (non-padding) inputs in the block
-            uint32_t r[kInputBlock];
-#pragma unroll
-            for (uint32_t j = 0; j < kInputBlock; ++j) r[j] = kRecInvalid;
-            if (pending) {
-                tested += nb;
-#pragma unroll
-                for (uint32_t j = 0; j < kInputBlock; ++j) r[j] = rec_plane[(i0 + j) * PQ + q];
-            }
-            if (lane == 0) slots += 64u * nb;
-            // Row probe: bits [row, row + ext0] of the unreached bitmask, the box's
-            // x-range at its middle y and lower θ, v (inside the box also when θ
-            // wraps). Escaped, dead and invalid records are not probed.
-            uint32_t row[kInputBlock], w0[kInputBlock], w1[kInputBlock];
-#pragma unroll
-            for (uint32_t j = 0; j < kInputBlock; ++j) {
-                const uint32_t rj = r[j];
-                const bool compact = (rj & (kRecEscape | kRecDead)) == 0;
-                const uint32_t px = lane_cell + rec_off0(rj);
-                const uint32_t py = (rec_off1(rj) + (rec_ext1(rj) >> 1)) * st1;
-                row[j] = compact ? px + py + probe_zw[i0 + j] : 0u;
-                w0[j] = ubits[row[j] >> 5];
-                w1[j] = ubits[(row[j] >> 5) + 1];
-            }
-            // Which pairs need the full count: a compact record the probe could
-            // not rule out, or an escaped record (never an invalid or dead one).
-            uint32_t need_mask = 0;
-#pragma unroll
-            for (uint32_t j = 0; j < kInputBlock; ++j) {
-                const uint32_t rj = r[j];
-                const bool compact = (rj & (kRecEscape | kRecDead)) == 0;
-                const uint64_t window = ((static_cast<uint64_t>(w1[j]) << 32) | w0[j]) >> (row[j] & 31u);
-                const uint64_t span = (2ull << rec_ext0(rj)) - 1ull;  // ext0 + 1 low bits
-                const bool blocked = compact && (window & span) != 0;
-                if (!blocked && rj != kRecInvalid && !(rj & kRecDead)) need_mask |= 1u << j;
-            }
-            // The count itself is rare: one non-unrolled pass over the block, in input
-            // order, so the first satisfied input wins. Records are re-read here
-            // rather than kept selectable in registers.
-            if (__ballot(need_mask != 0) != 0) {
-#pragma unroll 1
-                for (uint32_t j = 0; j < kInputBlock; ++j) {
-                    const bool need = ((need_mask >> j) & 1u) && pending;
-                    if (__ballot(need) == 0) continue;
-                    if (need) {
-                        uint32_t* const slot = rec_plane + (i0 + j) * PQ + q;
-                        const uint32_t i = i0 + j;
-                        const uint32_t iw = i / c.n_a;
-                        const uint2 b2 = A.box2[(plane % c.shape[2]) * c.n_w + iw];
-                        const uint2 b3 = A.box3[(plane / c.shape[2]) * c.n_a + (i - iw * c.n_a)];
-                        ++counted;
-                        if (count_test(A, *slot, ix, iy, b2, b3, slot)) {
-                            if (found == GSC_INVALID_INPUT) found = i;
-                            if (!A.sweep_all) pending = false;
-                        }
-                    }
-                }
-            }
-            if (!A.sweep_all && __ballot(pending) == 0) break;
-        }
-        // 3. update the state (reachability.hpp:93-105, kernels.cu:376-392).
-        if ((flags & kUse) && found != GSC_INVALID_INPUT) {
-            if (!(flags & kReached)) {
-                A.reach[s] = 1;
-                nr = 1;
-            }
-            if (!(flags & kCertified)) {
-                A.ctrl[s] = found;
-                if (flags & kCand) {
-                    nc = 1;
-                } else if (A.rstep[s] == GSC_UNREACHABLE_STEP) {
-                    A.rstep[s] = A.iter;
-                }
-            }
-        }
-    }
-    // Per-wave sums (32-bit is ample: a wave tests at most 64 * I pairs).
-    for (int off = 32; off > 0; off >>= 1) {
-        nr += __shfl_xor(nr, off, 64);
-        nc += __shfl_xor(nc, off, 64);
-        tested += __shfl_xor(tested, off, 64);
-        counted += __shfl_xor(counted, off, 64);
-    }
-    if (lane == 0) {
-        s_counts[wave][0] = nr;
-        s_counts[wave][1] = nc;
-        s_counts[wave][2] = tested;
-        s_counts[wave][3] = counted;
-        s_counts[wave][4] = slots;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long a = 0, b = 0, t = 0, n = 0, sl = 0;
-        for (int w = 0; w < 4; ++w) {
-            a += s_counts[w][0];
-            b += s_counts[w][1];
-            t += s_counts[w][2];
-            n += s_counts[w][3];
-            sl += s_counts[w][4];
-        }
-        IterCounters& slot = A.ctl->ring[A.iter % kRing];
-        const uint32_t k = blockIdx.x % kCounterSlots;
-        if (a) atomicAdd(&slot.new_reachable[k], a);
-        if (b) atomicAdd(&slot.new_certified[k], b);
-        if (t) atomicAdd(&slot.pairs_tested[k], t);
-        if (n) atomicAdd(&slot.boxes_counted[k], n);
-        if (sl) atomicAdd(&slot.test_slots[k], sl);
-    }
-}
-
-// Totals and the stop test of one iteration (kernels.cu:722-766). One wave.
-__global__ void finish_kernel(IterCtl* ctl, IterRecord* records, uint32_t iter, uint32_t max_iterations) {
-    if (blockIdx.x != 0 || threadIdx.x >= 64) return;
-    if (loop_done(ctl)) return;
+// Totals and the stop test of one iteration (kernels.cu:722-766), by one wave.
+// The iteration number lives on the device (ctl->iterations + 1), so a captured
+// sequence of launches replays correctly.
+__device__ __forceinline__ void finish_body(IterCtl* ctl, IterRecord* records, uint32_t max_iterations) {
+    const uint32_t iter = ctl->iterations + 1;
     IterCounters& slot = ctl->ring[iter % kRing];
     const uint32_t l = threadIdx.x;
     unsigned long long nr = 0, nc = 0, pt = 0, bc = 0, ts = 0;
@@ -719,6 +561,180 @@ __global__ void finish_kernel(IterCtl* ctl, IterRecord* records, uint32_t iter, 
     else if (iter >= max_iterations)
         done = kDoneMaxIterations;
     __atomic_store_n(&ctl->done, done, __ATOMIC_RELAXED);
+}
+
+// `done` and `iterations` are adjacent words of IterCtl: one 64-bit load gives
+// both (whether to run, and which iteration this is).
+__device__ __forceinline__ uint64_t loop_state(const IterCtl* ctl) {
+    static_assert(offsetof(IterCtl, iterations) == offsetof(IterCtl, done) + 4, "done, iterations adjacent");
+    return __atomic_load_n(reinterpret_cast<const unsigned long long*>(&ctl->done), __ATOMIC_RELAXED);
+}
+__device__ __forceinline__ bool state_done(uint64_t st) { return static_cast<uint32_t>(st) != 0; }
+__device__ __forceinline__ uint32_t state_next_iteration(uint64_t st) { return static_cast<uint32_t>(st >> 32) + 1u; }
+
+// ---------------------------------------------------------------------------
+// One instance per launch.
+//
+// Pair satisfaction fused with the first-input reduction, for one group of WG
+// cells of one (θ, v) plane, `block` = plane * groups_per_plane + group.
+//
+// The workgroup first compacts the states of its group that need testing into
+// LDS, so each wave runs 64 of them (no lanes spent on settled or invalid
+// states). Each wave then walks the inputs in increasing order in blocks of IB:
+// the block's record loads are issued together, then a bitmask probe per pair
+// reads the whole x-row of the successor box at its middle y (θ and v at their
+// lower ends) — any valid unreached cell there proves the pair unsatisfied —
+// and only the pairs the probe cannot rule out get the full prefix count. A
+// lane stops at its first satisfied input and the wave when no lane is left
+// (the reference breaks at the first satisfied input, reachability.hpp:106 /
+// kernels.cu:369-374). With sweep_all every pair of every state is tested, the
+// reference kernel's schedule (kernels.cu:668).
+// States are updated in place: the prefix snapshot was built before this
+// launch and only this workgroup touches these states (Jacobi, as kernels.cu:742).
+// The body is satisfy_body.inc.
+template <uint32_t WG, uint32_t IB>
+__global__ __launch_bounds__(WG) void satisfy_kernel(SatisfyArgs A) {
+    const uint64_t st = loop_state(A.ctl);
+    if (state_done(st)) return;
+    const uint32_t block = blockIdx.x;
+    const uint32_t iter = state_next_iteration(st);
+#include "satisfy_body.inc"
+}
+
+__global__ void finish_kernel(IterCtl* ctl, IterRecord* records, uint32_t max_iterations) {
+    if (blockIdx.x != 0 || threadIdx.x >= 64) return;
+    if (loop_done(ctl)) return;
+    finish_body(ctl, records, max_iterations);
+}
+
+// ---------------------------------------------------------------------------
+// Many instances per launch. Every stage of one iteration is a single launch
+// over the concatenated work of all instances: starts[k] .. starts[k + 1] - 1
+// are instance k's blocks (starts[n] = grid size), so the block -> instance
+// lookup is a binary search over n + 1 words, uniform per workgroup.
+__device__ __forceinline__ uint32_t find_instance(const uint32_t* __restrict__ starts, uint32_t n, uint32_t b) {
+    uint32_t lo = 0, hi = n;  // starts[lo] <= b < starts[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (__builtin_amdgcn_readfirstlane(starts[mid]) <= b)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void prefix_rows_batch_kernel(const BatchInstance* __restrict__ B,
+                                                                 const uint32_t* __restrict__ starts, uint32_t n) {
+    const uint32_t k = find_instance(starts, n, blockIdx.x);
+    const BatchInstance& I = B[k];
+    if (loop_done(I.sat.ctl)) return;
+    prefix_rows_body(I.sat.c, I.sat.valid, I.sat.reach, I.prefix_unreached, blockIdx.x - starts[k]);
+}
+
+__global__ __launch_bounds__(256) void prefix_axis_batch_kernel(const BatchInstance* __restrict__ B,
+                                                                 const uint32_t* __restrict__ starts, uint32_t n,
+                                                                 int axis) {
+    const uint32_t k = find_instance(starts, n, blockIdx.x);
+    const BatchInstance& I = B[k];
+    if (loop_done(I.sat.ctl)) return;
+    prefix_axis_body(I.sat.c, axis, I.prefix_unreached, blockIdx.x - starts[k]);
+}
+
+__global__ __launch_bounds__(256) void unreached_bits_batch_kernel(const BatchInstance* __restrict__ B,
+                                                                    const uint32_t* __restrict__ starts, uint32_t n) {
+    const uint32_t k = find_instance(starts, n, blockIdx.x);
+    const BatchInstance& I = B[k];
+    if (loop_done(I.sat.ctl)) return;
+    unreached_bits_body(I.sat.c.S, I.sat.valid, I.sat.reach, I.unreached_bits, blockIdx.x - starts[k]);
+}
+
+// The batched kernel reads its case's arguments from global memory; there the
+// function boundary gives the better register allocation (74 vs 97 VGPRs).
+template <uint32_t WG, uint32_t IB>
+__device__ __forceinline__ void satisfy_in_batch(const SatisfyArgs& A, uint32_t block, uint32_t iter) {
+#include "satisfy_body.inc"
+}
+
+template <uint32_t WG, uint32_t IB>
+__global__ __launch_bounds__(WG) void satisfy_batch_kernel(const BatchInstance* __restrict__ B,
+                                                           const uint32_t* __restrict__ starts, uint32_t n) {
+    const uint32_t inst = find_instance(starts, n, blockIdx.x);
+    const SatisfyArgs& A = B[inst].sat;
+    const uint64_t st = loop_state(A.ctl);
+    if (state_done(st)) return;
+    satisfy_in_batch<WG, IB>(A, blockIdx.x - starts[inst], state_next_iteration(st));
+}
+
+__global__ void finish_batch_kernel(const BatchInstance* __restrict__ B, uint32_t n, BatchStatus* status) {
+    const uint32_t k = blockIdx.x;
+    if (k >= n || threadIdx.x >= 64) return;
+    IterCtl* ctl = B[k].sat.ctl;
+    if (!loop_done(ctl)) finish_body(ctl, B[k].records, B[k].max_iterations);
+    if (threadIdx.x == 0) {
+        BatchStatus st;
+        st.done = __atomic_load_n(&ctl->done, __ATOMIC_RELAXED);
+        st.iterations = ctl->iterations;
+        st.total_reachable = ctl->total_reachable;
+        st.total_certified = ctl->total_certified;
+        status[k] = st;
+    }
+}
+
+// Seeds and control reset of every instance (seed_kernel + reset_ctl_kernel).
+__global__ __launch_bounds__(256) void seed_batch_kernel(const BatchInstance* __restrict__ B,
+                                                          const uint32_t* __restrict__ starts, uint32_t n) {
+    const uint32_t k = find_instance(starts, n, blockIdx.x);
+    const BatchInstance& I = B[k];
+    const uint32_t local = blockIdx.x - starts[k];
+    if (local == 0) {
+        reset_ctl_body(I.sat.ctl, I.candidate_count);
+        // max_iterations = 0 runs no iteration at all (reachability.hpp:53).
+        if (threadIdx.x == 0 && I.max_iterations == 0) I.sat.ctl->done = kDoneMaxIterations;
+    }
+    const uint32_t s = local * 256u + threadIdx.x;
+    if (s >= I.sat.c.S) return;
+    const uint8_t cd = I.sat.cand[s];
+    I.sat.reach[s] = cd;
+    I.sat.ctrl[s] = GSC_INVALID_INPUT;
+    I.sat.rstep[s] = cd ? 0u : GSC_UNREACHABLE_STEP;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers for the tile variants.
+namespace {
+using SatSingle = void (*)(SatisfyArgs);
+using SatBatch = void (*)(const BatchInstance*, const uint32_t*, uint32_t);
+
+template <uint32_t WG>
+SatSingle pick_single(uint32_t ib) {
+    return ib == 4 ? satisfy_kernel<WG, 4> : satisfy_kernel<WG, 8>;
+}
+template <uint32_t WG>
+SatBatch pick_batch(uint32_t ib) {
+    return ib == 4 ? satisfy_batch_kernel<WG, 4> : satisfy_batch_kernel<WG, 8>;
+}
+}  // namespace
+
+bool satisfy_variant_supported(uint32_t wg, uint32_t ib) {
+    return (wg == 128 || wg == 256 || wg == 512) && (ib == 4 || ib == 8);
+}
+
+uint32_t satisfy_blocks(const DevCase& c, uint32_t wg) {
+    return (c.PQ + wg - 1) / wg * c.shape[2] * c.shape[3];
+}
+
+hipError_t launch_satisfy(const SatisfyArgs& A, uint32_t wg, uint32_t ib, hipStream_t stream) {
+    const SatSingle f = wg == 128 ? pick_single<128>(ib) : wg == 512 ? pick_single<512>(ib) : pick_single<256>(ib);
+    hipLaunchKernelGGL(f, dim3(satisfy_blocks(A.c, wg)), dim3(wg), 0, stream, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_satisfy_batch(const BatchInstance* B, const uint32_t* starts, uint32_t n, uint32_t blocks,
+                                uint32_t wg, uint32_t ib, hipStream_t stream) {
+    const SatBatch f = wg == 128 ? pick_batch<128>(ib) : wg == 512 ? pick_batch<512>(ib) : pick_batch<256>(ib);
+    hipLaunchKernelGGL(f, dim3(blocks), dim3(wg), 0, stream, B, starts, n);
+    return hipGetLastError();
 }
 
 }  // namespace gsc_amd

@@ -1,6 +1,8 @@
 // kernels.hpp — kernel entry points and the device-side control block.
 #pragma once
 
+#include <cstddef>
+
 #include "gsc_device.hpp"
 
 namespace gsc_amd {
@@ -16,7 +18,9 @@ constexpr uint32_t kDoneMaxIterations = 3;  // "maximum iterations reached"
 // Workgroups add into one of kCounterSlots slots (by block index) so that tens
 // of thousands of workgroups do not serialise on a single address.
 constexpr uint32_t kCounterSlots = 32;
-// Inputs judged per block in satisfy_kernel (loads of a block are issued together).
+// Input padding: the record array holds I_pad = I rounded up to kInputBlock
+// inputs per (plane), so every satisfaction tile variant (4 or 8 inputs per
+// block of loads) reads whole blocks.
 constexpr uint32_t kInputBlock = 8;
 
 struct IterCounters {
@@ -78,14 +82,34 @@ struct SatisfyArgs {
     uint32_t* ctrl;
     uint32_t* rstep;
     IterCtl* ctl;
-    uint32_t iter;
     uint32_t sweep_all;
+};
+
+// One instance of a multi-instance (batched) iteration: its satisfaction
+// arguments plus what the prefix, bitmask, seed and finish stages write.
+struct BatchInstance {
+    SatisfyArgs sat;
+    uint32_t* prefix_unreached;
+    unsigned long long* unreached_bits;
+    IterRecord* records;
+    unsigned long long candidate_count;
+    uint32_t max_iterations;
+    uint32_t pad;
+};
+
+// Per-instance state published by finish_batch_kernel after every iteration.
+struct BatchStatus {
+    uint32_t done;
+    uint32_t iterations;
+    unsigned long long total_reachable;
+    unsigned long long total_certified;
 };
 
 __global__ void masks_kernel(DevCase c, const gsc_rect4* obstacles, const gsc_rect4* targets, uint8_t* valid,
                              uint8_t* cand, unsigned long long* counts);
 __global__ void seed_kernel(uint32_t S, const uint8_t* cand, uint8_t* reach, uint32_t* ctrl, uint32_t* rstep);
 __global__ void reset_ctl_kernel(IterCtl* ctl, unsigned long long candidate_count);
+__global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t value);
 __global__ void prefix_rows_kernel(DevCase c, const uint8_t* a, const uint8_t* b, uint32_t* P, const IterCtl* ctl);
 __global__ void prefix_axis_kernel(DevCase c, int axis, uint32_t* P, const IterCtl* ctl);
 __global__ void increments_kernel(DevCase c, const double2* cs, const double* vt, double2* inc);
@@ -96,7 +120,21 @@ __global__ void abstraction_kernel(AbstractionArgs A);
 __global__ void export_kernel(DevCase c, const uint32_t* rec, const uint2* box2, const uint2* box3,
                               const OverflowEntry* overflow, uint64_t pair_begin, uint64_t pair_end,
                               uint32_t* out_min, uint32_t* out_max, uint8_t* out_valid);
-__global__ void satisfy_kernel(SatisfyArgs A);
-__global__ void finish_kernel(IterCtl* ctl, IterRecord* records, uint32_t iter, uint32_t max_iterations);
+__global__ void finish_kernel(IterCtl* ctl, IterRecord* records, uint32_t max_iterations);
+
+// Satisfaction tile variants: WG cells (and threads) per workgroup, IB inputs per
+// block of record loads.
+bool satisfy_variant_supported(uint32_t wg, uint32_t ib);
+uint32_t satisfy_blocks(const DevCase& c, uint32_t wg);
+hipError_t launch_satisfy(const SatisfyArgs& A, uint32_t wg, uint32_t ib, hipStream_t stream);
+
+// Multi-instance stages: `starts` holds n + 1 block offsets per stage.
+__global__ void prefix_rows_batch_kernel(const BatchInstance* B, const uint32_t* starts, uint32_t n);
+__global__ void prefix_axis_batch_kernel(const BatchInstance* B, const uint32_t* starts, uint32_t n, int axis);
+__global__ void unreached_bits_batch_kernel(const BatchInstance* B, const uint32_t* starts, uint32_t n);
+__global__ void finish_batch_kernel(const BatchInstance* B, uint32_t n, BatchStatus* status);
+__global__ void seed_batch_kernel(const BatchInstance* B, const uint32_t* starts, uint32_t n);
+hipError_t launch_satisfy_batch(const BatchInstance* B, const uint32_t* starts, uint32_t n, uint32_t blocks,
+                                uint32_t wg, uint32_t ib, hipStream_t stream);
 
 }  // namespace gsc_amd

@@ -7,199 +7,8 @@
 // device-side `done` flag and returns at once after the stop test fired, so a
 // batch past the last iteration costs a few empty launches and the stop
 // iteration and message are exactly the reference's.
-#include <hip/hip_runtime.h>
+#include "solver.hpp"
 
-#include <algorithm>
-#include <chrono>
-#include <cmath>
-#include <cstdio>
-#include <cstring>
-#include <stdexcept>
-#include <string>
-#include <vector>
-
-#include "kernels.hpp"
-
-namespace gsc_amd {
-namespace {
-
-struct DeviceError : std::runtime_error {
-    hipError_t code;
-    DeviceError(hipError_t c, const std::string& what) : std::runtime_error(what), code(c) {}
-};
-struct ConfigError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
-#define GSC_HIP(expr)                                                                             \
-    do {                                                                                          \
-        hipError_t e__ = (expr);                                                                  \
-        if (e__ != hipSuccess) throw DeviceError(e__, hipGetErrorString(e__));                    \
-    } while (0)
-
-constexpr uint32_t kBatch = 8;          // iterations launched per host round trip
-constexpr uint32_t kEventsPerIter = 4;  // start, after prefix, after satisfy, after finish
-
-double ms_since(std::chrono::steady_clock::time_point t0) {
-    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-}
-
-void put(char* dst, size_t len, const std::string& s) {
-    if (dst && len) std::snprintf(dst, len, "%s", s.c_str());
-}
-
-uint32_t blocks_for(uint64_t items, uint32_t per_block) {
-    const uint64_t b = (items + per_block - 1) / per_block;
-    if (b > 0x7FFFFFFFull) throw ConfigError("kernel launch grid exceeds the device limit");
-    return static_cast<uint32_t>(b);
-}
-
-const char* done_message(uint32_t done) {
-    switch (done) {
-        case kDoneConverged: return "all candidate states obtained a controller";
-        case kDoneStalled: return "iteration stalled before candidate closure";
-        default: return "maximum iterations reached";
-    }
-}
-
-// build_state_grid / build_input_grid / build_prefix_layout (grid.hpp:103-168, 253-269),
-// with the reference's error texts.
-gsc_case_shape make_shape(const gsc_case_desc& c) {
-    gsc_case_shape sh;
-    std::memset(&sh, 0, sizeof(sh));
-    uint64_t total = 1;
-    for (int d = 0; d < 4; ++d) {
-        if (!(c.state_eta[d] > 0.0)) throw ConfigError("state eta must be positive");
-        const double width = c.state_ub[d] - c.state_lb[d];
-        if (!(width > 0.0)) throw ConfigError("state bounds must satisfy ub > lb");
-        sh.state_shape[d] = static_cast<uint32_t>(std::ceil(width / c.state_eta[d]));
-    }
-    sh.state_strides[0] = 1;
-    for (int d = 1; d < 4; ++d) sh.state_strides[d] = sh.state_strides[d - 1] * sh.state_shape[d - 1];
-    for (int d = 0; d < 4; ++d) total *= sh.state_shape[d];
-    sh.state_count = total;
-    if (total > 0xFFFFFFFFull) throw ConfigError("state grid exceeds the selected state index type");
-    for (int d = 0; d < 2; ++d) {
-        if (!(c.input_eta[d] > 0.0)) throw ConfigError("input eta must be positive");
-        const double width = c.input_ub[d] - c.input_lb[d];
-        if (!(width > 0.0)) throw ConfigError("input bounds must satisfy ub > lb");
-        sh.input_shape[d] = static_cast<uint32_t>(std::ceil(width / c.input_eta[d]));
-    }
-    sh.input_strides[0] = 1;
-    sh.input_strides[1] = sh.input_shape[0];
-    sh.input_count = static_cast<uint64_t>(sh.input_shape[0]) * sh.input_shape[1];
-    if (sh.input_count > 0xFFFFFFFFull) throw ConfigError("input grid exceeds uint32_t");
-    sh.pair_count = sh.state_count * sh.input_count;
-    uint64_t pt = 1;
-    for (int d = 0; d < 4; ++d) {
-        sh.prefix_shape[d] = sh.state_shape[d] + 1;
-        pt *= sh.prefix_shape[d];
-    }
-    sh.prefix_count = pt;
-    return sh;
-}
-
-// estimate_memory_budget, abstraction.hpp:211-243 (u32 state index, u64 prefix counts).
-gsc_memory_budget make_budget(const gsc_case_shape& sh) {
-    gsc_memory_budget b;
-    std::memset(&b, 0, sizeof(b));
-    const uint64_t P = sh.pair_count, S = sh.state_count;
-    b.pair_min_bytes = P * 4;
-    b.pair_max_bytes = P * 4;
-    b.pair_valid_bytes = P;
-    b.pair_satisfied_bytes = P;
-    b.reachable_prev_bytes = S;
-    b.reachable_next_bytes = S;
-    b.valid_mask_bytes = S;
-    b.candidate_mask_bytes = S;
-    b.controller_bytes = S * 4;
-    b.reach_step_bytes = S * 4;
-    b.prefix_reachable_bytes = sh.prefix_count * 8;
-    b.prefix_valid_bytes = sh.prefix_count * 8;
-    b.total_bytes = b.pair_min_bytes + b.pair_max_bytes + b.pair_valid_bytes + b.pair_satisfied_bytes +
-                    b.reachable_prev_bytes + b.reachable_next_bytes + b.valid_mask_bytes + b.candidate_mask_bytes +
-                    b.controller_bytes + b.reach_step_bytes + b.prefix_reachable_bytes + b.prefix_valid_bytes;
-    return b;
-}
-
-}  // namespace
-}  // namespace gsc_amd
-
-using namespace gsc_amd;
-
-struct gsc_solver {
-    gsc_solver_options opts{};
-    std::string name;
-    std::vector<gsc_rect4> obstacles, targets;
-    gsc_case_desc cfg{};
-    gsc_case_shape shape{};
-    DevCase dc{};
-    hipStream_t stream = nullptr;
-    std::vector<void*> allocations;
-    uint64_t bytes_allocated = 0;
-
-    uint8_t *valid = nullptr, *cand = nullptr, *reach = nullptr;
-    uint32_t *ctrl = nullptr, *rstep = nullptr;
-    uint32_t *prefix_valid = nullptr, *prefix_unreached = nullptr;
-    uint32_t* rec = nullptr;
-    OverflowEntry* overflow = nullptr;
-    uint32_t overflow_capacity = 0;
-    uint32_t* overflow_count = nullptr;
-    double2* cs = nullptr;
-    double* vt = nullptr;
-    double2* inc = nullptr;
-    unsigned long long* ubits = nullptr;
-    uint32_t* probe_zw = nullptr;
-    uint2 *box2 = nullptr, *box3 = nullptr;
-    double* rad = nullptr;
-    uint32_t* fail2d = nullptr;
-    gsc_rect4 *d_obstacles = nullptr, *d_targets = nullptr;
-    unsigned long long* mask_counts = nullptr;
-    IterCtl* ctl = nullptr;
-    IterRecord* records = nullptr;
-    IterCtl* host_ctl = nullptr;        // pinned
-    IterRecord* host_records = nullptr; // pinned
-    std::vector<hipEvent_t> events;
-
-    uint64_t candidate_count = 0;
-    bool abstraction_ready = false;
-    bool solved = false;
-
-    double abstraction_kernel_ms = 0.0;
-    double abstraction_wall_ms = 0.0;
-    double setup_h2d_ms = 0.0;
-    gsc_solve_summary summary{};
-    std::vector<gsc_iteration_stats> iter_stats;
-    gsc_kernel_timings timings{};
-    std::string last_error;
-
-    template <typename T>
-    T* alloc(uint64_t count) {
-        void* p = nullptr;
-        const uint64_t bytes = std::max<uint64_t>(count, 1) * sizeof(T);
-        GSC_HIP(hipMalloc(&p, bytes));
-        allocations.push_back(p);
-        bytes_allocated += bytes;
-        return static_cast<T*>(p);
-    }
-
-    ~gsc_solver() {
-        if (stream) (void)hipStreamSynchronize(stream);
-        for (void* p : allocations) (void)hipFree(p);
-        if (host_ctl) (void)hipHostFree(host_ctl);
-        if (host_records) (void)hipHostFree(host_records);
-        for (hipEvent_t e : events) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-
-    void build_tables(std::vector<double2>& h_cs, std::vector<double>& h_vt, std::vector<uint2>& h_box2,
-                      std::vector<uint2>& h_box3, std::vector<double>& h_rad, std::vector<uint32_t>& h_fail);
-    void setup();
-    void build_prefix(const uint8_t* a, const uint8_t* b, uint32_t* P, const IterCtl* ctl_or_null);
-    void build_abstraction();
-    void solve();
-    void download(const gsc_result_buffers* out, gsc_run_report* report);
-};
 
 // Host tables. The θ trajectory depends on (θ cell, ω) only and the v trajectory
 // on (v cell, a) only (model.hpp:35-45), so cos θ_k, sin θ_k and v_k are filled
@@ -334,7 +143,10 @@ void gsc_solver::setup() {
     dc.n_targets = static_cast<uint32_t>(targets.size());
 
     GSC_HIP(hipSetDevice(opts.device));
-    GSC_HIP(hipStreamCreate(&stream));
+    if (!stream) {
+        GSC_HIP(hipStreamCreate(&stream));
+        owns_stream = true;
+    }
     const uint64_t S = shape.state_count, PT = shape.prefix_count;
     valid = alloc<uint8_t>(S);
     cand = alloc<uint8_t>(S);
@@ -353,11 +165,9 @@ void gsc_solver::setup() {
     mask_counts = alloc<unsigned long long>(2);
     ctl = alloc<IterCtl>(1);
     records = alloc<IterRecord>(kRing);
-    GSC_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_ctl), sizeof(IterCtl), hipHostMallocDefault));
-    GSC_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_records), sizeof(IterRecord) * kRing, hipHostMallocDefault));
-    std::memset(host_ctl, 0, sizeof(IterCtl));
-    std::memset(host_records, 0, sizeof(IterRecord) * kRing);
-    events.resize(kBatch * kEventsPerIter + 2);
+    GSC_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_words), 4 * sizeof(uint32_t), hipHostMallocDefault));
+    std::memset(host_words, 0, 4 * sizeof(uint32_t));
+    events.resize(opts.time_kernels ? kBatch * kEventsPerIter + 2 : 2);
     for (auto& e : events) GSC_HIP(hipEventCreate(&e));
 
     std::vector<double2> h_cs;
@@ -408,8 +218,8 @@ void gsc_solver::setup() {
     hipLaunchKernelGGL(masks_kernel, dim3(blocks_for(S, 256)), dim3(256), 0, stream, dc, d_obstacles, d_targets, valid,
                        cand, mask_counts);
     GSC_HIP(hipGetLastError());
-    unsigned long long counts[2] = {0, 0};
-    GSC_HIP(hipMemcpyAsync(counts, mask_counts, sizeof(counts), hipMemcpyDeviceToHost, stream));
+    unsigned long long* counts = reinterpret_cast<unsigned long long*>(host_words);
+    GSC_HIP(hipMemcpyAsync(counts, mask_counts, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
     GSC_HIP(hipStreamSynchronize(stream));
     setup_h2d_ms = ms_since(t0);
     if (counts[1] != 0) throw ConfigError("candidate set contains invalid states");
@@ -433,42 +243,66 @@ void gsc_solver::build_prefix(const uint8_t* a, const uint8_t* b, uint32_t* P, c
 }
 
 void gsc_solver::build_abstraction() {
-    const auto t0 = std::chrono::steady_clock::now();
-    for (int attempt = 0; attempt < 2; ++attempt) {
-        GSC_HIP(hipMemsetAsync(overflow_count, 0, sizeof(uint32_t), stream));
-        AbstractionArgs A{};
-        A.c = dc;
-        A.inc = inc;
-        A.box2 = box2;
-        A.box3 = box3;
-        A.rad = rad;
-        A.fail2d = fail2d;
-        A.prefix_valid = prefix_valid;
-        A.rec = rec;
-        A.overflow = overflow;
-        A.overflow_count = overflow_count;
-        A.overflow_capacity = overflow_capacity;
-        GSC_HIP(hipEventRecord(events[0], stream));
-        hipLaunchKernelGGL(abstraction_kernel, dim3(blocks_for(dc.n_tasks, 4)), dim3(256), 0, stream, A);
-        GSC_HIP(hipGetLastError());
-        GSC_HIP(hipEventRecord(events[1], stream));
-        uint32_t used = 0;
-        GSC_HIP(hipMemcpyAsync(&used, overflow_count, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-        GSC_HIP(hipStreamSynchronize(stream));
-        float ms = 0.f;
-        GSC_HIP(hipEventElapsedTime(&ms, events[0], events[1]));
-        abstraction_kernel_ms = ms;
-        if (used <= overflow_capacity) break;
+    launch_abstraction();
+    finish_abstraction();
+}
+
+void gsc_solver::launch_abstraction() {
+    abstraction_t0 = std::chrono::steady_clock::now();
+    // A fill kernel, not hipMemsetAsync, so a caller may capture this sequence.
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, overflow_count, 1u, 0u);
+    GSC_HIP(hipGetLastError());
+    AbstractionArgs A{};
+    A.c = dc;
+    A.inc = inc;
+    A.box2 = box2;
+    A.box3 = box3;
+    A.rad = rad;
+    A.fail2d = fail2d;
+    A.prefix_valid = prefix_valid;
+    A.rec = rec;
+    A.overflow = overflow;
+    A.overflow_count = overflow_count;
+    A.overflow_capacity = overflow_capacity;
+    GSC_HIP(hipEventRecord(events[0], stream));
+    hipLaunchKernelGGL(abstraction_kernel, dim3(blocks_for(dc.n_tasks, 4)), dim3(256), 0, stream, A);
+    GSC_HIP(hipGetLastError());
+    GSC_HIP(hipEventRecord(events[1], stream));
+    GSC_HIP(hipMemcpyAsync(host_words, overflow_count, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+}
+
+void gsc_solver::finish_abstraction() {
+    GSC_HIP(hipStreamSynchronize(stream));
+    uint32_t used = host_words[0];
+    if (used > overflow_capacity) {
+        // More escaped boxes than the table holds: grow it and run once more.
         if (used >= kRecIndexMask) throw ConfigError("too many irregular successor boxes");
         overflow_capacity = used;
         overflow = alloc<OverflowEntry>(overflow_capacity);
+        const auto t0 = abstraction_t0;
+        launch_abstraction();
+        abstraction_t0 = t0;
+        GSC_HIP(hipStreamSynchronize(stream));
+        used = host_words[0];
+        if (used > overflow_capacity) throw ConfigError("successor-box overflow table did not converge");
     }
-    abstraction_wall_ms = ms_since(t0);
+    float ms = 0.f;
+    GSC_HIP(hipEventElapsedTime(&ms, events[0], events[1]));
+    abstraction_kernel_ms = ms;
+    abstraction_wall_ms = ms_since(abstraction_t0);
     abstraction_ready = true;
 }
 
 void gsc_solver::solve() {
     if (!abstraction_ready) build_abstraction();
+    if (!host_ctl) {
+        GSC_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_ctl), sizeof(IterCtl), hipHostMallocDefault));
+        std::memset(host_ctl, 0, sizeof(IterCtl));
+    }
+    if (!host_records) {
+        GSC_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_records), sizeof(IterRecord) * kRing, hipHostMallocDefault));
+        std::memset(host_records, 0, sizeof(IterRecord) * kRing);
+    }
     const uint32_t S = dc.S;
     hipLaunchKernelGGL(seed_kernel, dim3(blocks_for(S, 256)), dim3(256), 0, stream, S, cand, reach, ctrl, rstep);
     GSC_HIP(hipGetLastError());
@@ -497,7 +331,6 @@ void gsc_solver::solve() {
     A.rstep = rstep;
     A.ctl = ctl;
     A.sweep_all = opts.sweep_all_pairs ? 1u : 0u;
-    const uint32_t sat_blocks = dc.n_groups;  // one workgroup per 256-cell group of a plane
     const uint32_t max_it = cfg.max_iterations;
     uint32_t done = 0;
     uint32_t finished = 0;
@@ -513,11 +346,9 @@ void gsc_solver::solve() {
                                ubits, ctl);
             GSC_HIP(hipGetLastError());
             if (timed) GSC_HIP(hipEventRecord(ev[1], stream));
-            A.iter = it;
-            hipLaunchKernelGGL(satisfy_kernel, dim3(sat_blocks), dim3(256), 0, stream, A);
-            GSC_HIP(hipGetLastError());
+            GSC_HIP(launch_satisfy(A, tile_wg, tile_ib, stream));
             if (timed) GSC_HIP(hipEventRecord(ev[2], stream));
-            hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(64), 0, stream, ctl, records, it, max_it);
+            hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(64), 0, stream, ctl, records, max_it);
             GSC_HIP(hipGetLastError());
             if (timed) GSC_HIP(hipEventRecord(ev[3], stream));
         }
@@ -681,25 +512,6 @@ void gsc_solver::download(const gsc_result_buffers* out, gsc_run_report* report)
 // ===========================================================================
 // C ABI
 // ===========================================================================
-namespace {
-
-int status_of(const DeviceError& e) {
-    return e.code == hipErrorOutOfMemory || e.code == hipErrorMemoryAllocation ? GSC_ERR_OUT_OF_MEMORY : GSC_ERR_DEVICE;
-}
-
-void copy_case(gsc_solver* s, const gsc_case_desc& c) {
-    s->cfg = c;
-    s->name = c.name ? c.name : "";
-    s->obstacles.assign(c.obstacles, c.obstacles + (c.obstacles ? c.obstacle_count : 0));
-    s->targets.assign(c.extra_targets, c.extra_targets + (c.extra_targets ? c.extra_target_count : 0));
-    s->cfg.name = s->name.c_str();
-    s->cfg.obstacles = s->obstacles.empty() ? nullptr : s->obstacles.data();
-    s->cfg.obstacle_count = static_cast<uint32_t>(s->obstacles.size());
-    s->cfg.extra_targets = s->targets.empty() ? nullptr : s->targets.data();
-    s->cfg.extra_target_count = static_cast<uint32_t>(s->targets.size());
-}
-
-}  // namespace
 
 extern "C" {
 

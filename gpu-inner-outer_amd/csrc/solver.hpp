@@ -1,0 +1,317 @@
+// solver.hpp — one case resident on one device (gsc_solver) and the host-side
+// helpers shared by the single-instance, batched and serving front ends.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace gsc_amd {
+
+struct DeviceError : std::runtime_error {
+    hipError_t code;
+    DeviceError(hipError_t c, const std::string& what) : std::runtime_error(what), code(c) {}
+};
+struct ConfigError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define GSC_HIP(expr)                                                                             \
+    do {                                                                                          \
+        hipError_t e__ = (expr);                                                                  \
+        if (e__ != hipSuccess) throw DeviceError(e__, hipGetErrorString(e__));                    \
+    } while (0)
+
+constexpr uint32_t kBatch = 8;          // iterations launched per host round trip
+constexpr uint32_t kEventsPerIter = 4;  // start, after prefix, after satisfy, after finish
+
+inline double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+inline void put(char* dst, size_t len, const std::string& s) {
+    if (dst && len) std::snprintf(dst, len, "%s", s.c_str());
+}
+
+inline uint32_t blocks_for(uint64_t items, uint32_t per_block) {
+    const uint64_t b = (items + per_block - 1) / per_block;
+    if (b > 0x7FFFFFFFull) throw ConfigError("kernel launch grid exceeds the device limit");
+    return static_cast<uint32_t>(b);
+}
+
+inline const char* done_message(uint32_t done) {
+    switch (done) {
+        case kDoneConverged: return "all candidate states obtained a controller";
+        case kDoneStalled: return "iteration stalled before candidate closure";
+        default: return "maximum iterations reached";
+    }
+}
+
+// build_state_grid / build_input_grid / build_prefix_layout (grid.hpp:103-168, 253-269),
+// with the reference's error texts.
+inline gsc_case_shape make_shape(const gsc_case_desc& c) {
+    gsc_case_shape sh;
+    std::memset(&sh, 0, sizeof(sh));
+    uint64_t total = 1;
+    for (int d = 0; d < 4; ++d) {
+        if (!(c.state_eta[d] > 0.0)) throw ConfigError("state eta must be positive");
+        const double width = c.state_ub[d] - c.state_lb[d];
+        if (!(width > 0.0)) throw ConfigError("state bounds must satisfy ub > lb");
+        sh.state_shape[d] = static_cast<uint32_t>(std::ceil(width / c.state_eta[d]));
+    }
+    sh.state_strides[0] = 1;
+    for (int d = 1; d < 4; ++d) sh.state_strides[d] = sh.state_strides[d - 1] * sh.state_shape[d - 1];
+    for (int d = 0; d < 4; ++d) total *= sh.state_shape[d];
+    sh.state_count = total;
+    if (total > 0xFFFFFFFFull) throw ConfigError("state grid exceeds the selected state index type");
+    for (int d = 0; d < 2; ++d) {
+        if (!(c.input_eta[d] > 0.0)) throw ConfigError("input eta must be positive");
+        const double width = c.input_ub[d] - c.input_lb[d];
+        if (!(width > 0.0)) throw ConfigError("input bounds must satisfy ub > lb");
+        sh.input_shape[d] = static_cast<uint32_t>(std::ceil(width / c.input_eta[d]));
+    }
+    sh.input_strides[0] = 1;
+    sh.input_strides[1] = sh.input_shape[0];
+    sh.input_count = static_cast<uint64_t>(sh.input_shape[0]) * sh.input_shape[1];
+    if (sh.input_count > 0xFFFFFFFFull) throw ConfigError("input grid exceeds uint32_t");
+    sh.pair_count = sh.state_count * sh.input_count;
+    uint64_t pt = 1;
+    for (int d = 0; d < 4; ++d) {
+        sh.prefix_shape[d] = sh.state_shape[d] + 1;
+        pt *= sh.prefix_shape[d];
+    }
+    sh.prefix_count = pt;
+    return sh;
+}
+
+// estimate_memory_budget, abstraction.hpp:211-243 (u32 state index, u64 prefix counts).
+inline gsc_memory_budget make_budget(const gsc_case_shape& sh) {
+    gsc_memory_budget b;
+    std::memset(&b, 0, sizeof(b));
+    const uint64_t P = sh.pair_count, S = sh.state_count;
+    b.pair_min_bytes = P * 4;
+    b.pair_max_bytes = P * 4;
+    b.pair_valid_bytes = P;
+    b.pair_satisfied_bytes = P;
+    b.reachable_prev_bytes = S;
+    b.reachable_next_bytes = S;
+    b.valid_mask_bytes = S;
+    b.candidate_mask_bytes = S;
+    b.controller_bytes = S * 4;
+    b.reach_step_bytes = S * 4;
+    b.prefix_reachable_bytes = sh.prefix_count * 8;
+    b.prefix_valid_bytes = sh.prefix_count * 8;
+    b.total_bytes = b.pair_min_bytes + b.pair_max_bytes + b.pair_valid_bytes + b.pair_satisfied_bytes +
+                    b.reachable_prev_bytes + b.reachable_next_bytes + b.valid_mask_bytes + b.candidate_mask_bytes +
+                    b.controller_bytes + b.reach_step_bytes + b.prefix_reachable_bytes + b.prefix_valid_bytes;
+    return b;
+}
+
+// One hipMalloc'd block of HBM carved into 256-byte aligned pieces (first fit,
+// adjacent free pieces merged on release). The serving loop admits requests
+// against it so that no hipMalloc/hipFree runs while other solves are on the GPU.
+class DeviceArena {
+  public:
+    DeviceArena(int device, uint64_t bytes) : capacity_(round_up(bytes)) {
+        GSC_HIP(hipSetDevice(device));
+        GSC_HIP(hipMalloc(&base_, capacity_));
+        free_[0] = capacity_;
+    }
+    ~DeviceArena() {
+        if (base_) (void)hipFree(base_);
+    }
+    DeviceArena(const DeviceArena&) = delete;
+    DeviceArena& operator=(const DeviceArena&) = delete;
+
+    static uint64_t round_up(uint64_t b) { return (std::max<uint64_t>(b, 1) + 255) & ~uint64_t(255); }
+
+    // nullptr when no free piece is large enough.
+    void* allocate(uint64_t bytes) {
+        const uint64_t need = round_up(bytes);
+        std::lock_guard<std::mutex> lock(m_);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < need) continue;
+            const uint64_t off = it->first, size = it->second;
+            free_.erase(it);
+            if (size > need) free_[off + need] = size - need;
+            used_[off] = need;
+            in_use_ += need;
+            peak_ = std::max(peak_, in_use_);
+            return static_cast<char*>(base_) + off;
+        }
+        return nullptr;
+    }
+
+    void release(void* p) {
+        if (!p) return;
+        std::lock_guard<std::mutex> lock(m_);
+        const uint64_t off = static_cast<uint64_t>(static_cast<char*>(p) - static_cast<char*>(base_));
+        auto u = used_.find(off);
+        if (u == used_.end()) return;
+        uint64_t start = off, size = u->second;
+        in_use_ -= size;
+        used_.erase(u);
+        auto next = free_.find(start + size);
+        if (next != free_.end()) {
+            size += next->second;
+            free_.erase(next);
+        }
+        auto prev = free_.lower_bound(start);
+        if (prev != free_.begin()) {
+            --prev;
+            if (prev->first + prev->second == start) {
+                start = prev->first;
+                size += prev->second;
+                free_.erase(prev);
+            }
+        }
+        free_[start] = size;
+    }
+
+    uint64_t capacity() const { return capacity_; }
+    uint64_t in_use() {
+        std::lock_guard<std::mutex> lock(m_);
+        return in_use_;
+    }
+    uint64_t peak() {
+        std::lock_guard<std::mutex> lock(m_);
+        return peak_;
+    }
+
+  private:
+    void* base_ = nullptr;
+    uint64_t capacity_ = 0;
+    uint64_t in_use_ = 0;
+    uint64_t peak_ = 0;
+    std::map<uint64_t, uint64_t> free_;  // offset -> size
+    std::map<uint64_t, uint64_t> used_;  // offset -> size
+    std::mutex m_;
+};
+
+}  // namespace gsc_amd
+
+using namespace gsc_amd;
+
+struct gsc_solver {
+    gsc_solver_options opts{};
+    std::string name;
+    std::vector<gsc_rect4> obstacles, targets;
+    gsc_case_desc cfg{};
+    gsc_case_shape shape{};
+    DevCase dc{};
+    hipStream_t stream = nullptr;
+    bool owns_stream = true;
+    DeviceArena* arena = nullptr;  // when set, every device buffer comes from it
+    std::vector<void*> allocations;
+    uint64_t bytes_allocated = 0;
+
+    uint8_t *valid = nullptr, *cand = nullptr, *reach = nullptr;
+    uint32_t *ctrl = nullptr, *rstep = nullptr;
+    uint32_t *prefix_valid = nullptr, *prefix_unreached = nullptr;
+    uint32_t* rec = nullptr;
+    OverflowEntry* overflow = nullptr;
+    uint32_t overflow_capacity = 0;
+    uint32_t* overflow_count = nullptr;
+    double2* cs = nullptr;
+    double* vt = nullptr;
+    double2* inc = nullptr;
+    unsigned long long* ubits = nullptr;
+    uint32_t* probe_zw = nullptr;
+    uint2 *box2 = nullptr, *box3 = nullptr;
+    double* rad = nullptr;
+    uint32_t* fail2d = nullptr;
+    gsc_rect4 *d_obstacles = nullptr, *d_targets = nullptr;
+    unsigned long long* mask_counts = nullptr;
+    IterCtl* ctl = nullptr;
+    IterRecord* records = nullptr;
+    IterCtl* host_ctl = nullptr;        // pinned, allocated by the first solve()
+    IterRecord* host_records = nullptr; // pinned, allocated by the first solve()
+    uint32_t* host_words = nullptr;     // pinned scratch words (overflow count, mask counts)
+    std::vector<hipEvent_t> events;
+
+    uint64_t candidate_count = 0;
+    // Satisfaction tile: cells per workgroup and inputs per block of record loads.
+    uint32_t tile_wg = 256, tile_ib = 8;
+    bool abstraction_ready = false;
+    bool solved = false;
+
+    std::chrono::steady_clock::time_point abstraction_t0{};
+    double abstraction_kernel_ms = 0.0;
+    double abstraction_wall_ms = 0.0;
+    double setup_h2d_ms = 0.0;
+    gsc_solve_summary summary{};
+    std::vector<gsc_iteration_stats> iter_stats;
+    gsc_kernel_timings timings{};
+    std::string last_error;
+
+    template <typename T>
+    T* alloc(uint64_t count) {
+        void* p = nullptr;
+        const uint64_t bytes = std::max<uint64_t>(count, 1) * sizeof(T);
+        if (arena) {
+            p = arena->allocate(bytes);
+            if (!p) throw DeviceError(hipErrorOutOfMemory, "device arena exhausted");
+        } else {
+            GSC_HIP(hipMalloc(&p, bytes));
+        }
+        allocations.push_back(p);
+        bytes_allocated += bytes;
+        return static_cast<T*>(p);
+    }
+
+    ~gsc_solver() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void* p : allocations) {
+            if (arena)
+                arena->release(p);
+            else
+                (void)hipFree(p);
+        }
+        if (host_ctl) (void)hipHostFree(host_ctl);
+        if (host_records) (void)hipHostFree(host_records);
+        if (host_words) (void)hipHostFree(host_words);
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        if (stream && owns_stream) (void)hipStreamDestroy(stream);
+    }
+
+    // Device bytes setup() allocates for a case of this shape (the arena
+    // admission estimate; the overflow table may grow later).
+    static uint64_t device_bytes(const gsc_case_shape& sh, uint32_t substeps);
+
+    void build_tables(std::vector<double2>& h_cs, std::vector<double>& h_vt, std::vector<uint2>& h_box2,
+                      std::vector<uint2>& h_box3, std::vector<double>& h_rad, std::vector<uint32_t>& h_fail);
+    void setup();
+    void build_prefix(const uint8_t* a, const uint8_t* b, uint32_t* P, const IterCtl* ctl_or_null);
+    void build_abstraction();     // launch_abstraction() + finish_abstraction()
+    void launch_abstraction();    // enqueue on `stream`, no wait
+    void finish_abstraction();    // wait; rerun once with a larger overflow table if needed
+    void solve();
+    void download(const gsc_result_buffers* out, gsc_run_report* report);
+};
+
+inline int status_of(const DeviceError& e) {
+    return e.code == hipErrorOutOfMemory || e.code == hipErrorMemoryAllocation ? GSC_ERR_OUT_OF_MEMORY : GSC_ERR_DEVICE;
+}
+
+inline void copy_case(gsc_solver* s, const gsc_case_desc& c) {
+    s->cfg = c;
+    s->name = c.name ? c.name : "";
+    s->obstacles.assign(c.obstacles, c.obstacles + (c.obstacles ? c.obstacle_count : 0));
+    s->targets.assign(c.extra_targets, c.extra_targets + (c.extra_targets ? c.extra_target_count : 0));
+    s->cfg.name = s->name.c_str();
+    s->cfg.obstacles = s->obstacles.empty() ? nullptr : s->obstacles.data();
+    s->cfg.obstacle_count = static_cast<uint32_t>(s->obstacles.size());
+    s->cfg.extra_targets = s->targets.empty() ? nullptr : s->targets.data();
+    s->cfg.extra_target_count = static_cast<uint32_t>(s->targets.size());
+}

@@ -221,6 +221,41 @@ class SolveSummary(ctypes.Structure):
     ]
 
 
+class BatchOptions(ctypes.Structure):
+    _fields_ = [
+        ("device", c_int32),
+        ("tile_wg", c_int32),
+        ("tile_ib", c_int32),
+        ("iterations_per_launch", c_int32),
+        ("order", c_int32),
+        ("use_graph", c_int32),
+        ("reserved", c_int32 * 2),
+    ]
+
+
+class BatchSummary(ctypes.Structure):
+    _fields_ = [
+        ("cases", c_uint32),
+        ("replays", c_uint32),
+        ("max_iterations", c_uint32),
+        ("stopped", c_uint32),
+        ("pairs", c_uint64),
+        ("abstraction_ms", c_double),
+        ("solve_ms", c_double),
+        ("gpu_ms", c_double),
+    ]
+
+
+class BatchTuneEntry(ctypes.Structure):
+    _fields_ = [
+        ("tile_wg", c_int32),
+        ("tile_ib", c_int32),
+        ("iterations_per_launch", c_int32),
+        ("order", c_int32),
+        ("solve_ms", c_double),
+    ]
+
+
 # Every exported symbol of include/gsc_amd.h: (name, restype, argtypes).
 SYMBOLS = [
     ("gsc_backend_compiled", c_int, []),
@@ -240,6 +275,18 @@ SYMBOLS = [
     ("gsc_config_desc", POINTER(CaseDesc), [c_void_p]),
     ("gsc_config_free", None, [c_void_p]),
     ("gsc_config_builtin", c_int, [c_int, POINTER(c_void_p)]),
+    ("gsc_batch_default_options", None, [POINTER(BatchOptions)]),
+    ("gsc_batch_create", c_int, [POINTER(POINTER(CaseDesc)), c_uint32, POINTER(BatchOptions), POINTER(c_void_p),
+                                 c_char_p, c_size_t]),
+    ("gsc_batch_build_abstraction", c_int, [c_void_p, POINTER(c_double)]),
+    ("gsc_batch_solve", c_int, [c_void_p, POINTER(BatchSummary)]),
+    ("gsc_batch_tune", c_int, [c_void_p, POINTER(BatchTuneEntry), c_uint32, POINTER(c_uint32)]),
+    ("gsc_batch_get_options", c_int, [c_void_p, POINTER(BatchOptions)]),
+    ("gsc_batch_set_options", c_int, [c_void_p, POINTER(BatchOptions)]),
+    ("gsc_batch_case_summary", c_int, [c_void_p, c_uint32, POINTER(SolveSummary)]),
+    ("gsc_batch_download", c_int, [c_void_p, c_uint32, POINTER(ResultBuffers), POINTER(RunReport)]),
+    ("gsc_batch_last_error", c_char_p, [c_void_p]),
+    ("gsc_batch_destroy", None, [c_void_p]),
 ]
 
 _lib = None

@@ -300,6 +300,67 @@ void gsc_config_free(gsc_config* c);
  * (config.cpp:129-241); which = 0..3. */
 int gsc_config_builtin(int which, gsc_config** out);
 
+/* ---------------------------------------------------------------------------
+ * Batched solves. No reference counterpart: the reference solves one case per
+ * run_case_cuda_u32 call (kernels.cu:492-875). A batch holds many independent
+ * cases of uneven grid and input sizes resident on one device and advances all
+ * of them together: every stage of an iteration (prefix build, bitmask,
+ * satisfaction + reduction, stop test) is one launch over the concatenated
+ * work of all unfinished cases, and `iterations_per_launch` such iterations
+ * are captured once in a HIP graph that is replayed until every case has
+ * stopped. Each case stops on its own rule and keeps its own results, which are
+ * identical to a single-case solve of it.
+ * ------------------------------------------------------------------------- */
+typedef struct gsc_batch gsc_batch;
+
+typedef struct gsc_batch_options {
+    int32_t device;
+    int32_t tile_wg;               /* cells (and threads) per satisfaction workgroup: 128, 256, 512 */
+    int32_t tile_ib;               /* inputs per block of record loads: 4 or 8 */
+    int32_t iterations_per_launch; /* iterations per graph replay, 1..64 */
+    int32_t order;                 /* 0: cases in the given order, 1: largest pair count first */
+    int32_t use_graph;             /* 1: replay a captured HIP graph, 0: launch directly */
+    int32_t reserved[2];
+} gsc_batch_options;
+
+typedef struct gsc_batch_summary {
+    uint32_t cases;
+    uint32_t replays;          /* graph replays (or direct launch rounds) of the last solve */
+    uint32_t max_iterations;   /* largest iteration count over the cases */
+    uint32_t stopped;          /* cases that reached a stop rule */
+    uint64_t pairs;            /* total state-input pairs */
+    double abstraction_ms;     /* wall time of the last gsc_batch_build_abstraction */
+    double solve_ms;           /* wall time of the last gsc_batch_solve (seeds to last stop) */
+    double gpu_ms;             /* device time of the replays (HIP events) */
+} gsc_batch_summary;
+
+typedef struct gsc_batch_tune_entry {
+    int32_t tile_wg;
+    int32_t tile_ib;
+    int32_t iterations_per_launch;
+    int32_t order;
+    double solve_ms; /* best of the timed solves of this configuration */
+} gsc_batch_tune_entry;
+
+void gsc_batch_default_options(gsc_batch_options* opts);
+/* Prepare every case (masks, tables, device buffers). Fails with the first
+ * case's configuration error, naming its index. */
+int gsc_batch_create(const gsc_case_desc* const* cases, uint32_t count, const gsc_batch_options* opts,
+                     gsc_batch** out, char* err, size_t err_len);
+int gsc_batch_build_abstraction(gsc_batch* b, double* wall_ms);
+/* Seeds every case and iterates all of them to their stop rules. */
+int gsc_batch_solve(gsc_batch* b, gsc_batch_summary* out);
+/* Times each tile / schedule configuration on this batch (full solves, results
+ * unchanged), keeps the fastest and reports every one tried. */
+int gsc_batch_tune(gsc_batch* b, gsc_batch_tune_entry* entries, uint32_t capacity, uint32_t* written);
+int gsc_batch_get_options(const gsc_batch* b, gsc_batch_options* out);
+int gsc_batch_set_options(gsc_batch* b, const gsc_batch_options* opts);
+/* Per-case results of the last solve, as gsc_solver_solve / gsc_solver_download. */
+int gsc_batch_case_summary(gsc_batch* b, uint32_t index, gsc_solve_summary* out);
+int gsc_batch_download(gsc_batch* b, uint32_t index, const gsc_result_buffers* out, gsc_run_report* report);
+const char* gsc_batch_last_error(const gsc_batch* b);
+void gsc_batch_destroy(gsc_batch* b);
+
 #ifdef __cplusplus
 }
 #endif
