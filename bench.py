@@ -25,6 +25,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "gpu-inner-outer_amd", "python"))
 
@@ -144,7 +146,7 @@ def _pmc_pass(counter, child_args, outdir, timeout_s=150):
         for row in csv.DictReader(f):
             if row["Counter_Name"] != counter:
                 continue
-            k = row["Kernel_Name"].split("(")[0].split("::")[-1]
+            k = row["Kernel_Name"].split("(")[0].split("<")[0].split("::")[-1]  # bare kernel name
             n, s = per_kernel.get(k, (0, 0.0))
             per_kernel[k] = (n + 1, s + float(row["Counter_Value"]))
     return child, per_kernel
@@ -153,7 +155,8 @@ def _pmc_pass(counter, child_args, outdir, timeout_s=150):
 def measure_traffic(workload_name, kernel, sweep_all, outdir):
     """HBM bytes per launch of `kernel`: (FETCH_SIZE x f_fetch + WRITE_SIZE x f_write) x 1024,
     f = known copy bytes / counter reading of the copy kernel, averaged over the work launches."""
-    run_args = ["--workload", workload_name, "--steps", "1", "--warmup", "0", "--no-cpu-baseline", "--traffic", "off"]
+    run_args = ["--workload", workload_name, "--steps", "1", "--warmup", "0", "--no-cpu-baseline", "--traffic", "off",
+                "--no-batching", "--no-serving"]
     if sweep_all:
         run_args.append("--sweep-all-pairs")
     out = {}
@@ -177,6 +180,106 @@ def measure_traffic(workload_name, kernel, sweep_all, outdir):
     return out
 
 
+def _same_result(a, b):
+    keys = ("valid_mask", "candidate_mask", "reachable_mask", "controller", "reach_step")
+    return all(np.array_equal(getattr(a, k), getattr(b, k)) for k in keys) and a.iterations == b.iterations \
+        and a.result_message == b.result_message
+
+
+def batching_section(count=32, seed=3, steps=3):
+    """32 uneven reach-avoid cases: plain single-case solves one after another
+    versus one batch (fused stages, captured graph, autotuned tiles). A step is
+    abstraction + fixpoint of every case; setup (tables, masks, allocation) is
+    outside both timings."""
+    from gsc_amd import cases
+    from gsc_amd.batch import Batch
+    from gsc_amd.solver import Solver
+
+    cs = cases.batch_instances(count, seed=seed)
+    print(f"[bench] batching: {count} cases, {sum(c.state_count() * c.input_count() for c in cs)} pairs",
+          file=sys.stderr, flush=True)
+    solvers = [Solver(c, time_kernels=False) for c in cs]
+    for s in solvers:
+        s.build_abstraction()
+        s.solve()
+    seq = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        for s in solvers:
+            s.build_abstraction()
+            s.solve()
+        seq.append((time.perf_counter() - t0) * 1000.0)
+    b = Batch(cs)
+    b.build_abstraction()
+    t0 = time.perf_counter()
+    tried = b.tune()
+    tune_s = time.perf_counter() - t0
+    bat, summ = [], None
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        b.build_abstraction()
+        summ = b.solve()
+        bat.append((time.perf_counter() - t0) * 1000.0)
+    identical = all(_same_result(b.download(k), solvers[k].download()) for k in range(count))
+    opts = b.options
+    b.close()
+    for s in solvers:
+        s.close()
+    seq_ms, bat_ms = sum(seq) / steps, sum(bat) / steps
+    return {
+        "workload": f"batch_instances(count={count}, seed={seed}): cells log-uniform in [5e4, 1e6], extents ~9:10:4:1, "
+                    "inputs in {16, 36, 64, 100}, 1-3 targets, 0-5 obstacles",
+        "cases": count,
+        "pairs": int(sum(c.state_count() * c.input_count() for c in cs)),
+        "batched_ms_per_step": bat_ms,
+        "sequential_ms_per_step": seq_ms,
+        "instances_per_s": count / (bat_ms / 1000.0),
+        "sequential_instances_per_s": count / (seq_ms / 1000.0),
+        "speedup": seq_ms / bat_ms,
+        "graph_replays": summ["replays"],
+        "max_iterations": summ["max_iterations"],
+        "tuned": {k: opts[k] for k in ("tile_wg", "tile_ib", "iterations_per_launch", "order")},
+        "tune_configs_tried": len(tried),
+        "tune_s": tune_s,
+        "identical_to_single_case_solves": identical,
+    }
+
+
+def serving_section(count=48, seed=5, rates=(100.0, 400.0, 1600.0)):
+    """Poisson streams of solve requests (cells log-uniform in [2e4, 4e6]) at
+    several rates: the continuous-batching server versus the plain per-call path
+    serving the same stream one request at a time."""
+    from gsc_amd import cases
+    from gsc_amd.serve import Server, poisson_load, sequential_load
+
+    reqs = cases.serving_requests(count, seed=seed)
+    out = []
+    identical = None
+    for i, rate in enumerate(rates):
+        keep = i == 0
+        print(f"[bench] serving: {count} requests at {rate:g}/s", file=sys.stderr, flush=True)
+        with Server(max_active=64, iterations_per_step=4, hbm_budget_bytes=96 << 30) as srv:
+            served, r_srv = poisson_load(srv, reqs, rate, seed=int(rate), keep_results=keep)
+        seq, r_seq = sequential_load(reqs, rate, seed=int(rate), keep_results=keep)
+        if keep:
+            identical = all(_same_result(a, b) for a, b in zip(r_srv, r_seq))
+            del r_srv, r_seq
+        out.append({
+            "rate_per_s": rate,
+            "served": {k: served[k] for k in ("solves_per_s", "p50_ms", "p99_ms", "mean_ms", "max_ms")},
+            "server": {k: served["server"][k] for k in ("max_active_seen", "steps", "budget_peak")},
+            "failed": len(served["failed"]),
+            "sequential": {k: seq[k] for k in ("solves_per_s", "p50_ms", "p99_ms", "mean_ms", "max_ms")},
+        })
+    return {
+        "workload": f"serving_requests(count={count}, seed={seed}): cells log-uniform in [2e4, 4e6], "
+                    "inputs in {16, 36, 64, 100}, 1-3 targets, 0-5 obstacles; Poisson arrivals",
+        "server": "max_active 64, 4 iterations per step, 96 GiB HBM budget",
+        "rates": out,
+        "identical_to_plain_path": identical,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,6 +294,8 @@ def main():
                     help="measure the dominant kernel's HBM bytes with two rocprofv3 --pmc passes")
     ap.add_argument("--pmc-dir", default=os.path.join(ROOT, "gpurun_out", "bench_pmc"))
     ap.add_argument("--copy-calibration", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--no-batching", action="store_true", help="skip the 32-case batching comparison")
+    ap.add_argument("--no-serving", action="store_true", help="skip the Poisson serving comparison")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -294,6 +399,10 @@ def main():
             roof["traffic_detail"] = t
         except Exception as e:  # counters unavailable: traffic stays null, the reason is kept
             roof["traffic_error"] = str(e)[:300]
+    if not args.no_batching:
+        line["batching"] = batching_section()
+    if not args.no_serving:
+        line["serving"] = serving_section()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(case, args.cpu_budget)
     print(json.dumps(line))

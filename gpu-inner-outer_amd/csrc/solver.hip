@@ -165,7 +165,10 @@ void gsc_solver::setup() {
     mask_counts = alloc<unsigned long long>(2);
     ctl = alloc<IterCtl>(1);
     records = alloc<IterRecord>(kRing);
-    GSC_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_words), 4 * sizeof(uint32_t), hipHostMallocDefault));
+    if (!host_words) {
+        GSC_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_words), 4 * sizeof(uint32_t), hipHostMallocDefault));
+        owns_host_words = true;
+    }
     std::memset(host_words, 0, 4 * sizeof(uint32_t));
     events.resize(opts.time_kernels ? kBatch * kEventsPerIter + 2 : 2);
     for (auto& e : events) GSC_HIP(hipEventCreate(&e));
@@ -227,6 +230,28 @@ void gsc_solver::setup() {
 
     build_prefix(valid, nullptr, prefix_valid, nullptr);
 
+}
+
+uint64_t gsc_solver::device_bytes(const gsc_case_shape& sh, uint32_t substeps) {
+    auto r = [](uint64_t b) { return DeviceArena::round_up(b); };
+    const uint64_t S = sh.state_count, PT = sh.prefix_count;
+    const uint64_t n0 = sh.state_shape[0], n1 = sh.state_shape[1], n2 = sh.state_shape[2], n3 = sh.state_shape[3];
+    const uint64_t na = sh.input_shape[0], nw = sh.input_shape[1], I = sh.input_count;
+    const uint64_t I_pad = (I + kInputBlock - 1) / kInputBlock * kInputBlock;
+    const uint64_t planes = n2 * n3, K = substeps;
+    uint64_t b = 3 * r(S) + 2 * r(4 * S) + 2 * r(4 * PT);
+    b += r(4 * planes * I_pad * n0 * n1) + r(4 * planes * I_pad) + r(8 * ((S + 63) / 64 + 1));
+    b += r(sizeof(OverflowEntry) * (1u << 16)) + r(4) + r(16) + r(sizeof(IterCtl)) + r(sizeof(IterRecord) * kRing);
+    b += r(16 * n2 * nw * K) + r(8 * n3 * na * K) + r(8 * n2 * nw) + r(8 * n3 * na) + r(16 * n3);
+    b += r(4 * (n0 + 1) * (n1 + 1)) + r(16 * planes * I * K) + 2 * r(64 * 64);
+    return b;
+}
+
+void gsc_solver::seed_async() {
+    hipLaunchKernelGGL(seed_kernel, dim3(blocks_for(dc.S, 256)), dim3(256), 0, stream, dc.S, cand, reach, ctrl, rstep);
+    GSC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(reset_ctl_kernel, dim3(1), dim3(256), 0, stream, ctl, static_cast<unsigned long long>(candidate_count));
+    GSC_HIP(hipGetLastError());
 }
 
 void gsc_solver::build_prefix(const uint8_t* a, const uint8_t* b, uint32_t* P, const IterCtl* c) {

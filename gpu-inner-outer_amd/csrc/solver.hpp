@@ -238,6 +238,7 @@ struct gsc_solver {
     IterCtl* host_ctl = nullptr;        // pinned, allocated by the first solve()
     IterRecord* host_records = nullptr; // pinned, allocated by the first solve()
     uint32_t* host_words = nullptr;     // pinned scratch words (overflow count, mask counts)
+    bool owns_host_words = true;        // false when a server lends them from its pool
     std::vector<hipEvent_t> events;
 
     uint64_t candidate_count = 0;
@@ -280,7 +281,7 @@ struct gsc_solver {
         }
         if (host_ctl) (void)hipHostFree(host_ctl);
         if (host_records) (void)hipHostFree(host_records);
-        if (host_words) (void)hipHostFree(host_words);
+        if (host_words && owns_host_words) (void)hipHostFree(host_words);
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
         if (stream && owns_stream) (void)hipStreamDestroy(stream);
     }
@@ -296,6 +297,7 @@ struct gsc_solver {
     void build_abstraction();     // launch_abstraction() + finish_abstraction()
     void launch_abstraction();    // enqueue on `stream`, no wait
     void finish_abstraction();    // wait; rerun once with a larger overflow table if needed
+    void seed_async();            // fixpoint seeds and control reset, enqueued on `stream`
     void solve();
     void download(const gsc_result_buffers* out, gsc_run_report* report);
 };

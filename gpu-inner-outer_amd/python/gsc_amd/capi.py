@@ -256,6 +256,52 @@ class BatchTuneEntry(ctypes.Structure):
     ]
 
 
+class ServerOptions(ctypes.Structure):
+    _fields_ = [
+        ("device", c_int32),
+        ("max_active", c_int32),
+        ("iterations_per_step", c_int32),
+        ("tile_wg", c_int32),
+        ("tile_ib", c_int32),
+        ("reserved", c_int32 * 3),
+        ("hbm_budget_bytes", c_uint64),
+    ]
+
+
+REQUEST_QUEUED, REQUEST_RUNNING, REQUEST_DONE, REQUEST_FAILED = 0, 1, 2, 3
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [
+        ("id", c_uint64),
+        ("state", c_int32),
+        ("status", c_int32),
+        ("state_count", c_uint64),
+        ("pair_count", c_uint64),
+        ("device_bytes", c_uint64),
+        ("queued_ms", c_double),
+        ("latency_ms", c_double),
+        ("summary", SolveSummary),
+        ("error", c_char * MESSAGE_LEN),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", c_uint64),
+        ("completed", c_uint64),
+        ("failed", c_uint64),
+        ("steps", c_uint64),
+        ("active", c_uint32),
+        ("queued", c_uint32),
+        ("max_active_seen", c_uint32),
+        ("reserved", c_uint32),
+        ("budget_bytes", c_uint64),
+        ("budget_in_use", c_uint64),
+        ("budget_peak", c_uint64),
+    ]
+
+
 # Every exported symbol of include/gsc_amd.h: (name, restype, argtypes).
 SYMBOLS = [
     ("gsc_backend_compiled", c_int, []),
@@ -287,6 +333,13 @@ SYMBOLS = [
     ("gsc_batch_download", c_int, [c_void_p, c_uint32, POINTER(ResultBuffers), POINTER(RunReport)]),
     ("gsc_batch_last_error", c_char_p, [c_void_p]),
     ("gsc_batch_destroy", None, [c_void_p]),
+    ("gsc_server_default_options", None, [POINTER(ServerOptions)]),
+    ("gsc_server_create", c_int, [POINTER(ServerOptions), POINTER(c_void_p), c_char_p, c_size_t]),
+    ("gsc_server_submit", c_int, [c_void_p, POINTER(CaseDesc), POINTER(c_uint64)]),
+    ("gsc_server_wait", c_int, [c_void_p, c_uint64, c_double, POINTER(RequestInfo)]),
+    ("gsc_server_fetch", c_int, [c_void_p, c_uint64, POINTER(ResultBuffers), POINTER(RunReport)]),
+    ("gsc_server_get_stats", c_int, [c_void_p, POINTER(ServerStats)]),
+    ("gsc_server_destroy", None, [c_void_p]),
 ]
 
 _lib = None

@@ -361,6 +361,77 @@ int gsc_batch_download(gsc_batch* b, uint32_t index, const gsc_result_buffers* o
 const char* gsc_batch_last_error(const gsc_batch* b);
 void gsc_batch_destroy(gsc_batch* b);
 
+/* ---------------------------------------------------------------------------
+ * Serving. No reference counterpart (the reference has no request loop). A
+ * server owns one device and an HBM budget carved out once at creation; a
+ * worker thread admits submitted cases in arrival order while the budget and
+ * `max_active` allow, prepares each one (masks, tables, abstraction) while the
+ * others keep iterating, and advances every admitted case together: each step
+ * runs `iterations_per_step` fused iterations over all of them (continuous
+ * batching). A case leaves the device as soon as it stops; its masks,
+ * controller, reach steps and iteration history are copied to host memory and
+ * its HBM returns to the budget. Results are identical to a single-case solve.
+ * ------------------------------------------------------------------------- */
+typedef struct gsc_server gsc_server;
+
+typedef struct gsc_server_options {
+    int32_t device;
+    int32_t max_active;           /* cases iterating together, 1..1024 */
+    int32_t iterations_per_step;  /* fused iterations between admission checks, 1..64 */
+    int32_t tile_wg;              /* satisfaction tile, as gsc_batch_options */
+    int32_t tile_ib;
+    int32_t reserved[3];
+    uint64_t hbm_budget_bytes;    /* device memory reserved for admitted cases */
+} gsc_server_options;
+
+typedef enum gsc_request_state {
+    GSC_REQUEST_QUEUED = 0,
+    GSC_REQUEST_RUNNING = 1,
+    GSC_REQUEST_DONE = 2,
+    GSC_REQUEST_FAILED = 3
+} gsc_request_state;
+
+typedef struct gsc_request_info {
+    uint64_t id;
+    int32_t state;               /* gsc_request_state */
+    int32_t status;              /* GSC_OK, or the error of a failed request */
+    uint64_t state_count;
+    uint64_t pair_count;
+    uint64_t device_bytes;       /* HBM admitted against the budget */
+    double queued_ms;            /* submit -> admitted */
+    double latency_ms;           /* submit -> results on the host */
+    gsc_solve_summary summary;   /* as gsc_solver_solve (solve_ms = admitted -> stopped) */
+    char error[GSC_MESSAGE_LEN];
+} gsc_request_info;
+
+typedef struct gsc_server_stats {
+    uint64_t submitted;
+    uint64_t completed;
+    uint64_t failed;
+    uint64_t steps;              /* fused iteration steps run */
+    uint32_t active;             /* cases on the device now */
+    uint32_t queued;
+    uint32_t max_active_seen;    /* most cases iterating together */
+    uint32_t reserved;
+    uint64_t budget_bytes;
+    uint64_t budget_in_use;
+    uint64_t budget_peak;
+} gsc_server_stats;
+
+void gsc_server_default_options(gsc_server_options* opts);
+int gsc_server_create(const gsc_server_options* opts, gsc_server** out, char* err, size_t err_len);
+/* Thread-safe; returns at once with the request id. */
+int gsc_server_submit(gsc_server* s, const gsc_case_desc* cfg, uint64_t* id);
+/* Waits up to timeout_ms (< 0: forever) for the request to finish; GSC_ERR_BUSY
+ * on timeout. `info` may be NULL. */
+int gsc_server_wait(gsc_server* s, uint64_t id, double timeout_ms, gsc_request_info* info);
+/* Copies a finished request's state arrays and iteration history (any pointer
+ * of `out` may be NULL) and forgets the request. */
+int gsc_server_fetch(gsc_server* s, uint64_t id, const gsc_result_buffers* out, gsc_run_report* report);
+int gsc_server_get_stats(gsc_server* s, gsc_server_stats* out);
+/* Stops admitting, lets running cases finish, joins the worker and frees the device. */
+void gsc_server_destroy(gsc_server* s);
+
 #ifdef __cplusplus
 }
 #endif

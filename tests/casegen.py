@@ -94,3 +94,31 @@ def assert_same_abstraction(gpu, ref, label=""):
         if not np.array_equal(a, b):
             bad = np.flatnonzero(a != b)
             raise AssertionError(f"{label}: {k_gpu} differs at {len(bad)} pairs, first {bad[:5]} gpu={a[bad[:5]]} ref={b[bad[:5]]}")
+
+
+def uneven_cases(oracle, n_small: int = 8, n_random: int = 4):
+    """Cases of uneven grid and input sizes for the batched and served paths:
+    valid small random cases (configuration errors skipped), 16..100-input
+    instances with ~9:10:4:1 grids, the 30x30x24x10 case and a case with
+    max_iterations = 0."""
+    from gsc_amd import cases as gcases
+
+    out = []
+    for seed in range(60):
+        c = small_random_case(seed)
+        try:
+            oracle.masks(c)
+        except ValueError:
+            continue
+        out.append(c)
+        if len(out) == n_small:
+            break
+    rng = np.random.default_rng(11)
+    for k in range(n_random):
+        out.append(gcases.random_instance(rng, 3e3, 2e4, f"uneven_{k}"))
+    out.append(gcases.unicycle_30x30x24x10(sample_time=0.5))
+    z = small_random_case(3)
+    z.name = "zero_iterations"
+    z.max_iterations = 0
+    out.append(z)
+    return out

@@ -8,6 +8,7 @@
 import ctypes
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -42,6 +43,42 @@ def test_struct_sizes_match_c_layout():
     assert ctypes.sizeof(capi.IterationStats) == 8 + 4 * 8 + 4 * 8
     assert ctypes.sizeof(capi.KernelTimings) == 15 * 8
     assert ctypes.sizeof(capi.MemoryBudget) == 13 * 8
+
+
+C_NAMES = {
+    "Rect4": "gsc_rect4", "CaseDesc": "gsc_case_desc", "CaseShape": "gsc_case_shape",
+    "IterationStats": "gsc_iteration_stats", "KernelTimings": "gsc_kernel_timings",
+    "MemoryStats": "gsc_memory_stats", "MemoryBudget": "gsc_memory_budget", "RunReport": "gsc_run_report",
+    "ResultBuffers": "gsc_result_buffers", "SolverOptions": "gsc_solver_options",
+    "SolveSummary": "gsc_solve_summary", "BatchOptions": "gsc_batch_options", "BatchSummary": "gsc_batch_summary",
+    "BatchTuneEntry": "gsc_batch_tune_entry", "ServerOptions": "gsc_server_options",
+    "RequestInfo": "gsc_request_info", "ServerStats": "gsc_server_stats",
+}
+
+
+def test_every_struct_layout_matches_the_header(tmp_path):
+    """sizeof and every field offset of each ctypes mirror against the C
+    compiler's view of include/gsc_amd.h."""
+    structs = {n: getattr(capi, n) for n in C_NAMES}
+    missing = [n for n, v in vars(capi).items()
+               if isinstance(v, type) and issubclass(v, ctypes.Structure) and n not in C_NAMES]
+    assert not missing, f"structs without a C name: {missing}"
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "gsc_amd.h"', "int main(void) {"]
+    for py, c in C_NAMES.items():
+        lines.append(f'printf("{py} %zu\\n", sizeof({c}));')
+        for f, _ in structs[py]._fields_:
+            lines.append(f'printf("{py}.{f} %zu\\n", offsetof({c}, {f}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    got = dict(line.rsplit(" ", 1) for line in out.splitlines())
+    for py, cls in structs.items():
+        assert int(got[py]) == ctypes.sizeof(cls), f"sizeof {py}"
+        for f, _ in cls._fields_:
+            assert int(got[f"{py}.{f}"]) == getattr(cls, f).offset, f"{py}.{f}"
 
 
 def test_backend_compiled():

@@ -4,7 +4,7 @@ schedule configuration the autotuner may pick, on repeated graph replays."""
 import numpy as np
 import pytest
 
-from casegen import assert_same_solution, small_random_case
+from casegen import assert_same_solution, uneven_cases
 from gsc_amd import cases
 from gsc_amd.batch import Batch
 from gsc_amd.solver import GscError
@@ -12,31 +12,9 @@ from gsc_amd.solver import GscError
 pytestmark = pytest.mark.gpu
 
 
-def _uneven_cases(oracle):
-    out = []
-    for seed in range(40):
-        c = small_random_case(seed)
-        try:
-            oracle.masks(c)
-        except ValueError:
-            continue  # configuration errors are covered by the single-case tests
-        out.append(c)
-        if len(out) == 8:
-            break
-    rng = np.random.default_rng(11)
-    for k in range(4):  # 16..100 inputs, ~9:10:4:1 grids
-        out.append(cases.random_instance(rng, 3e3, 2e4, f"uneven_{k}"))
-    out.append(cases.unicycle_30x30x24x10(sample_time=0.5))
-    z = small_random_case(3)
-    z.name = "zero_iterations"
-    z.max_iterations = 0
-    out.append(z)
-    return out
-
-
 @pytest.fixture(scope="module")
 def batch_and_refs(gpu, oracle):
-    cs = _uneven_cases(oracle)
+    cs = uneven_cases(oracle)
     refs = [oracle.run(c, keep_abstraction=False) for c in cs]
     b = Batch(cs)
     b.build_abstraction()
