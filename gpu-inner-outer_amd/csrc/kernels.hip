@@ -312,13 +312,75 @@ __global__ __launch_bounds__(256) void probe_zw_kernel(DevCase c, const uint2* _
     probe_zw[t] = v;
 }
 
-// Abstraction. Per pair, the arithmetic is the reference's in the reference's order:
-//   centre           grid.hpp:42
+// Successor cell box of one pair, the reference's arithmetic in the reference's order:
+//   centre           grid.hpp:42 (x0, y0 passed in)
 //   nominal x, y     model.hpp:35-42: x += dt*(v_k*cos θ_k), y += dt*(v_k*sin θ_k),
 //                    the increments read from increments_kernel's table
 //   radius           model.hpp:71-73, 84-89, 107-111 (per v cell, host table)
 //   cells            grid.hpp:181-251 (dims 0,1 here; dims 2,3 from box2/box3)
 //   constraint scan  abstraction.hpp:110-159 as a 2D prefix query over x,y fails
+// Returns false for an invalid pair (abstraction.hpp:187-205). `inc` points at
+// the pair's (plane, input) row of K increments and must be wave-uniform.
+__device__ __forceinline__ bool pair_box(const DevCase& c, ConstPtr<double> inc, uint2 b2, uint2 b3, double x0,
+                                         double y0, double rad0, double rad1, const uint32_t* __restrict__ fail2d,
+                                         uint32_t lo[4], uint32_t hi[4]) {
+    if (b2.x == kBoxReject || b3.x == kBoxReject) return false;
+    lo[2] = b2.x;
+    hi[2] = b2.y;
+    lo[3] = b3.x;
+    hi[3] = b3.y;
+    // Scalar loads, eight steps per wait; the additions stay strictly in step order.
+    double px = x0, py = y0;
+    uint32_t k = 0;
+    for (; k + 8 <= c.K; k += 8) {
+        double tx[8], ty[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            tx[j] = inc[2 * (k + j)];
+            ty[j] = inc[2 * (k + j) + 1];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            px = px + tx[j];
+            py = py + ty[j];
+        }
+    }
+    for (; k < c.K; ++k) {
+        px = px + inc[2 * k];
+        py = py + inc[2 * k + 1];
+    }
+    if (!dim_to_cells(c, 0, px - rad0, px + rad0, &lo[0], &hi[0]) ||
+        !dim_to_cells(c, 1, py - rad1, py + rad1, &lo[1], &hi[1]))
+        return false;
+    if (c.constraint_type != GSC_CONSTRAINT_NONE) {
+        // Every loop of abstraction.hpp:124-156 must be non-empty for a cell to be
+        // tested; the test reads x and y only, so it reduces to "no failing (x, y)
+        // cell in the x/y rectangle". The θ walk is empty only when its range is
+        // inverted without a wrap (abstraction.hpp:125-135).
+        const int wd = c.wrap_dim;
+        const bool has_wrap = lo[wd] > hi[wd];
+        const bool empty = lo[3] > hi[3] || (lo[2] > hi[2] && !has_wrap) || lo[1] > hi[1] || lo[0] > hi[0];
+        if (!empty) {
+            const uint32_t fpitch = c.shape[0] + 1;
+            const uint32_t f = fail2d[(hi[1] + 1) * fpitch + hi[0] + 1] - fail2d[(hi[1] + 1) * fpitch + lo[0]] -
+                               fail2d[lo[1] * fpitch + hi[0] + 1] + fail2d[lo[1] * fpitch + lo[0]];
+            if (f != 0) return false;
+        }
+    }
+    return true;
+}
+
+// The state's own centre must satisfy the constraint (abstraction.hpp:177-181).
+__device__ __forceinline__ bool state_constraint_ok(const DevCase& c, const uint32_t* __restrict__ fail2d, uint32_t ix,
+                                                    uint32_t iy) {
+    if (c.constraint_type == GSC_CONSTRAINT_NONE) return true;
+    const uint32_t fpitch = c.shape[0] + 1;
+    const uint32_t f = fail2d[(iy + 1) * fpitch + ix + 1] - fail2d[(iy + 1) * fpitch + ix] -
+                       fail2d[iy * fpitch + ix + 1] + fail2d[iy * fpitch + ix];
+    return f == 0;
+}
+
+// Abstraction: every pair's box, packed into its record (see gsc_device.hpp).
 // The `dead` bit of compact records is left 0 here and settled lazily by the
 // satisfaction pass (see satisfy_kernel); escaped pairs get it eagerly.
 __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
@@ -336,15 +398,7 @@ __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
     const uint32_t ix = live ? q - iy * c.shape[0] : 0;
     const double x0 = cell_center(c.lb[0], ix, c.eta[0]);
     const double y0 = cell_center(c.lb[1], iy, c.eta[1]);
-    const uint32_t fpitch = c.shape[0] + 1;
-    bool state_ok = live;
-    if (live && c.constraint_type != GSC_CONSTRAINT_NONE) {
-        // abstraction.hpp:177-181: the state's own centre must satisfy the constraint.
-        const uint32_t* F = A.fail2d;
-        const uint32_t f = F[(iy + 1) * fpitch + ix + 1] - F[(iy + 1) * fpitch + ix] - F[iy * fpitch + ix + 1] +
-                           F[iy * fpitch + ix];
-        state_ok = f == 0;
-    }
+    const bool state_ok = live && state_constraint_ok(c, A.fail2d, ix, iy);
     const double rad0 = A.rad[2 * iv];
     const double rad1 = A.rad[2 * iv + 1];
     const int wd = c.wrap_dim;
@@ -358,53 +412,9 @@ __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
             const uint2 b3 = A.box3[iv * c.n_a + ia];
             uint32_t r = kRecInvalid;
             uint32_t lo[4], hi[4];
-            bool ok = state_ok && b2.x != kBoxReject && b3.x != kBoxReject;
-            if (ok) {
-                lo[2] = b2.x;
-                hi[2] = b2.y;
-                lo[3] = b3.x;
-                hi[3] = b3.y;
-                // The increments are wave-uniform: read them through the constant
-                // address space (scalar loads), eight steps per wait. The additions
-                // stay strictly in step order.
-                const ConstPtr<double> inc =
-                    to_const(reinterpret_cast<const double*>(A.inc + (static_cast<size_t>(plane) * c.I + i) * c.K));
-                double px = x0, py = y0;
-                uint32_t k = 0;
-                for (; k + 8 <= c.K; k += 8) {
-                    double tx[8], ty[8];
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        tx[j] = inc[2 * (k + j)];
-                        ty[j] = inc[2 * (k + j) + 1];
-                    }
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        px = px + tx[j];
-                        py = py + ty[j];
-                    }
-                }
-                for (; k < c.K; ++k) {
-                    px = px + inc[2 * k];
-                    py = py + inc[2 * k + 1];
-                }
-                ok = dim_to_cells(c, 0, px - rad0, px + rad0, &lo[0], &hi[0]) &&
-                     dim_to_cells(c, 1, py - rad1, py + rad1, &lo[1], &hi[1]);
-            }
-            if (ok && c.constraint_type != GSC_CONSTRAINT_NONE) {
-                // Every loop of abstraction.hpp:124-156 must be non-empty for a cell to be
-                // tested; the test reads x and y only, so it reduces to "no failing (x, y)
-                // cell in the x/y rectangle". The θ walk is empty only when its range is
-                // inverted without a wrap (abstraction.hpp:125-135).
-                const bool has_wrap = lo[wd] > hi[wd];
-                const bool empty = lo[3] > hi[3] || (lo[2] > hi[2] && !has_wrap) || lo[1] > hi[1] || lo[0] > hi[0];
-                if (!empty) {
-                    const uint32_t* F = A.fail2d;
-                    const uint32_t f = F[(hi[1] + 1) * fpitch + hi[0] + 1] - F[(hi[1] + 1) * fpitch + lo[0]] -
-                                       F[lo[1] * fpitch + hi[0] + 1] + F[lo[1] * fpitch + lo[0]];
-                    ok = f == 0;
-                }
-            }
+            const ConstPtr<double> inc =
+                to_const(reinterpret_cast<const double*>(A.inc + (static_cast<size_t>(plane) * c.I + i) * c.K));
+            const bool ok = state_ok && pair_box(c, inc, b2, b3, x0, y0, rad0, rad1, A.fail2d, lo, hi);
             if (ok) {
                 // Inverted ranges on a non-wrap axis or wrapped x/y ranges take the escape path.
                 const bool regular = lo[0] <= hi[0] && lo[1] <= hi[1] && (lo[2] <= hi[2] || wd == 2) &&
@@ -599,6 +609,163 @@ __global__ __launch_bounds__(WG) void satisfy_kernel(SatisfyArgs A) {
     const uint32_t block = blockIdx.x;
     const uint32_t iter = state_next_iteration(st);
 #include "satisfy_body.inc"
+}
+
+// Pair satisfaction + first-input reduction with the successor boxes recomputed
+// in FP64 (pair_box, the abstraction kernel's arithmetic) instead of read from
+// stored records: no per-pair HBM at all, at the cost of the K trajectory steps
+// per tested pair. Same schedule and results as satisfy_kernel; one input at a
+// time per lane, lanes leave at their first satisfied input.
+__global__ __launch_bounds__(256) void satisfy_recompute_kernel(RecomputeArgs R) {
+    const SatisfyArgs& A = R.sat;
+    const uint64_t st = loop_state(A.ctl);
+    if (state_done(st)) return;
+    const uint32_t iter = state_next_iteration(st);
+    const DevCase& c = A.c;
+    constexpr uint32_t kUse = 1, kCand = 2, kReached = 4, kCertified = 8;
+    __shared__ uint32_t s_q[256];
+    __shared__ uint8_t s_flags[256];
+    __shared__ uint32_t s_wave_n[4];
+    __shared__ uint32_t s_counts[4][5];
+    const uint32_t lane = lane_id();
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t PQ = c.PQ;
+    const uint32_t gpp = (PQ + 255u) / 256u;
+    const uint32_t plane = blockIdx.x / gpp;
+    const uint32_t g0 = (blockIdx.x - plane * gpp) * 256u;
+    {
+        const uint32_t q = g0 + threadIdx.x;
+        const uint32_t s = plane * PQ + q;
+        uint32_t f = 0;
+        bool eval = false;
+        if (q < PQ) {
+            if (A.valid[s]) {
+                const bool is_cand = A.cand[s] != 0;
+                const bool reached = A.reach[s] != 0;
+                const bool certified = A.ctrl[s] != GSC_INVALID_INPUT;
+                const bool use = !((!is_cand && reached) || (is_cand && certified));
+                f = (use ? kUse : 0) | (is_cand ? kCand : 0) | (reached ? kReached : 0) | (certified ? kCertified : 0);
+            }
+            eval = A.sweep_all ? true : (f & kUse) != 0;
+        }
+        const uint64_t m = __ballot(eval);
+        if (lane == 0) s_wave_n[wave] = static_cast<uint32_t>(__popcll(m));
+        __syncthreads();
+        uint32_t base = 0;
+        for (uint32_t w = 0; w < wave; ++w) base += s_wave_n[w];
+        if (eval) {
+            const uint32_t pos = base + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
+            s_q[pos] = q;
+            s_flags[pos] = static_cast<uint8_t>(f);
+        }
+        __syncthreads();
+    }
+    const uint32_t n_eval = s_wave_n[0] + s_wave_n[1] + s_wave_n[2] + s_wave_n[3];
+    uint32_t nr = 0, nc = 0, tested = 0, counted = 0, slots = 0;
+    const uint32_t e = wave * 64u + lane;
+    const bool has = e < n_eval;
+    if (wave * 64u < n_eval) {
+        const uint32_t q = has ? s_q[e] : g0;
+        const uint32_t flags = has ? s_flags[e] : 0u;
+        const uint32_t s = plane * PQ + q;
+        const uint32_t iy = q / c.shape[0];
+        const uint32_t ix = q - iy * c.shape[0];
+        const uint32_t iz = plane % c.shape[2];
+        const uint32_t iv = plane / c.shape[2];
+        const double x0 = cell_center(c.lb[0], ix, c.eta[0]);
+        const double y0 = cell_center(c.lb[1], iy, c.eta[1]);
+        const double rad0 = R.rad[2 * iv];
+        const double rad1 = R.rad[2 * iv + 1];
+        const bool state_ok = state_constraint_ok(c, R.fail2d, ix, iy);
+        const uint32_t* const ubits = reinterpret_cast<const uint32_t*>(A.unreached_bits);
+        const int wd = c.wrap_dim;
+        uint32_t found = GSC_INVALID_INPUT;
+        bool pending = has;
+        uint32_t i = 0;
+        for (uint32_t iw = 0; iw < c.n_w; ++iw) {
+            const uint2 b2 = A.box2[iz * c.n_w + iw];
+            for (uint32_t ia = 0; ia < c.n_a; ++ia, ++i) {
+                if (__ballot(pending) == 0) break;
+                if (lane == 0) slots += 64u;
+                if (!pending) continue;
+                ++tested;
+                const uint2 b3 = A.box3[iv * c.n_a + ia];
+                const ConstPtr<double> inc =
+                    to_const(reinterpret_cast<const double*>(R.inc + (static_cast<size_t>(plane) * c.I + i) * c.K));
+                uint32_t lo[4], hi[4];
+                if (!state_ok || !pair_box(c, inc, b2, b3, x0, y0, rad0, rad1, R.fail2d, lo, hi)) continue;
+                const bool regular = lo[0] <= hi[0] && lo[1] <= hi[1] && (lo[2] <= hi[2] || wd == 2) &&
+                                     (lo[3] <= hi[3] || wd == 3);
+                bool sat;
+                if (regular) {
+                    // Row probe as in satisfy_kernel (x-range at middle y, lower θ and v).
+                    if (hi[0] - lo[0] <= kExtMax) {
+                        const uint32_t row = lo[0] + (lo[1] + ((hi[1] - lo[1]) >> 1)) * c.stride[1] +
+                                             lo[2] * c.stride[2] + lo[3] * c.stride[3];
+                        const uint64_t window =
+                            ((static_cast<uint64_t>(ubits[(row >> 5) + 1]) << 32) | ubits[row >> 5]) >> (row & 31u);
+                        if ((window & ((2ull << (hi[0] - lo[0])) - 1ull)) != 0) continue;
+                    }
+                    ++counted;
+                    sat = box_count_wrap_u32(A.prefix_unreached, c.pstride, wd, c.shape[wd], lo, hi) == 0 &&
+                          box_count_wrap_u32(A.prefix_valid, c.pstride, wd, c.shape[wd], lo, hi) != 0;
+                } else {
+                    ++counted;
+                    sat = box_count_i64(A.prefix_valid, c.pstride, wd, c.shape[wd], lo, hi) != 0 &&
+                          box_count_i64(A.prefix_unreached, c.pstride, wd, c.shape[wd], lo, hi) == 0;
+                }
+                if (sat) {
+                    if (found == GSC_INVALID_INPUT) found = i;
+                    if (!A.sweep_all) pending = false;
+                }
+            }
+        }
+        if ((flags & kUse) && found != GSC_INVALID_INPUT) {
+            if (!(flags & kReached)) {
+                A.reach[s] = 1;
+                nr = 1;
+            }
+            if (!(flags & kCertified)) {
+                A.ctrl[s] = found;
+                if (flags & kCand) {
+                    nc = 1;
+                } else if (A.rstep[s] == GSC_UNREACHABLE_STEP) {
+                    A.rstep[s] = iter;
+                }
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        nr += __shfl_xor(nr, off, 64);
+        nc += __shfl_xor(nc, off, 64);
+        tested += __shfl_xor(tested, off, 64);
+        counted += __shfl_xor(counted, off, 64);
+    }
+    if (lane == 0) {
+        s_counts[wave][0] = nr;
+        s_counts[wave][1] = nc;
+        s_counts[wave][2] = tested;
+        s_counts[wave][3] = counted;
+        s_counts[wave][4] = slots;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long a = 0, b = 0, t = 0, n = 0, sl = 0;
+        for (uint32_t w = 0; w < 4; ++w) {
+            a += s_counts[w][0];
+            b += s_counts[w][1];
+            t += s_counts[w][2];
+            n += s_counts[w][3];
+            sl += s_counts[w][4];
+        }
+        IterCounters& slot = A.ctl->ring[iter % kRing];
+        const uint32_t k = blockIdx.x % kCounterSlots;
+        if (a) atomicAdd(&slot.new_reachable[k], a);
+        if (b) atomicAdd(&slot.new_certified[k], b);
+        if (t) atomicAdd(&slot.pairs_tested[k], t);
+        if (n) atomicAdd(&slot.boxes_counted[k], n);
+        if (sl) atomicAdd(&slot.test_slots[k], sl);
+    }
 }
 
 __global__ void finish_kernel(IterCtl* ctl, IterRecord* records, uint32_t max_iterations) {

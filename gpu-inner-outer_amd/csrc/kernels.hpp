@@ -85,6 +85,15 @@ struct SatisfyArgs {
     uint32_t sweep_all;
 };
 
+// Satisfaction without stored pair records: every tested pair's successor box
+// is recomputed from the trajectory tables in the same launch.
+struct RecomputeArgs {
+    SatisfyArgs sat;       // rec is unused
+    const double2* inc;    // as AbstractionArgs
+    const double* rad;
+    const uint32_t* fail2d;
+};
+
 // One instance of a multi-instance (batched) iteration: its satisfaction
 // arguments plus what the prefix, bitmask, seed and finish stages write.
 struct BatchInstance {
@@ -127,6 +136,7 @@ __global__ void finish_kernel(IterCtl* ctl, IterRecord* records, uint32_t max_it
 bool satisfy_variant_supported(uint32_t wg, uint32_t ib);
 uint32_t satisfy_blocks(const DevCase& c, uint32_t wg);
 hipError_t launch_satisfy(const SatisfyArgs& A, uint32_t wg, uint32_t ib, hipStream_t stream);
+__global__ void satisfy_recompute_kernel(RecomputeArgs R);
 
 // Multi-instance stages: `starts` holds n + 1 block offsets per stage.
 __global__ void prefix_rows_batch_kernel(const BatchInstance* B, const uint32_t* starts, uint32_t n);

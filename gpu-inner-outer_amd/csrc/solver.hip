@@ -156,10 +156,11 @@ void gsc_solver::setup() {
     prefix_valid = alloc<uint32_t>(PT);
     prefix_unreached = alloc<uint32_t>(PT);
     const uint64_t n_planes = static_cast<uint64_t>(shape.state_shape[2]) * shape.state_shape[3];
-    rec = alloc<uint32_t>(n_planes * dc.I_pad * dc.PQ);
+    // Recompute mode keeps no per-pair state at all.
+    if (!opts.recompute_abstraction) rec = alloc<uint32_t>(n_planes * dc.I_pad * dc.PQ);
     probe_zw = alloc<uint32_t>(n_planes * dc.I_pad);
     ubits = alloc<unsigned long long>((S + 63) / 64 + 1);  // +1: the row probe reads one word past
-    overflow_capacity = 1u << 16;
+    overflow_capacity = opts.recompute_abstraction ? 1u : 1u << 16;
     overflow = alloc<OverflowEntry>(overflow_capacity);
     overflow_count = alloc<uint32_t>(1);
     mask_counts = alloc<unsigned long long>(2);
@@ -268,6 +269,12 @@ void gsc_solver::build_prefix(const uint8_t* a, const uint8_t* b, uint32_t* P, c
 }
 
 void gsc_solver::build_abstraction() {
+    if (opts.recompute_abstraction) {  // boxes are recomputed inside every satisfaction launch
+        abstraction_kernel_ms = 0.0;
+        abstraction_wall_ms = 0.0;
+        abstraction_ready = true;
+        return;
+    }
     launch_abstraction();
     finish_abstraction();
 }
@@ -356,6 +363,11 @@ void gsc_solver::solve() {
     A.rstep = rstep;
     A.ctl = ctl;
     A.sweep_all = opts.sweep_all_pairs ? 1u : 0u;
+    RecomputeArgs RA{};
+    RA.sat = A;
+    RA.inc = inc;
+    RA.rad = rad;
+    RA.fail2d = fail2d;
     const uint32_t max_it = cfg.max_iterations;
     uint32_t done = 0;
     uint32_t finished = 0;
@@ -371,7 +383,12 @@ void gsc_solver::solve() {
                                ubits, ctl);
             GSC_HIP(hipGetLastError());
             if (timed) GSC_HIP(hipEventRecord(ev[1], stream));
-            GSC_HIP(launch_satisfy(A, tile_wg, tile_ib, stream));
+            if (opts.recompute_abstraction) {
+                hipLaunchKernelGGL(satisfy_recompute_kernel, dim3(satisfy_blocks(dc, 256)), dim3(256), 0, stream, RA);
+                GSC_HIP(hipGetLastError());
+            } else {
+                GSC_HIP(launch_satisfy(A, tile_wg, tile_ib, stream));
+            }
             if (timed) GSC_HIP(hipEventRecord(ev[2], stream));
             hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(64), 0, stream, ctl, records, max_it);
             GSC_HIP(hipGetLastError());
@@ -461,6 +478,8 @@ void gsc_solver::download(const gsc_result_buffers* out, gsc_run_report* report)
         }
         GSC_HIP(hipStreamSynchronize(stream));
         if (abstraction_ready && (out->abstraction_min_flat || out->abstraction_max_flat || out->abstraction_valid)) {
+            if (opts.recompute_abstraction)
+                throw ConfigError("the pair abstraction is not stored in recompute mode");
             // Expand the records to the reference triplet in slices through device staging.
             const uint64_t P = shape.pair_count;
             const uint64_t slice = std::min<uint64_t>(P, 1ull << 26);

@@ -197,7 +197,8 @@ class SolverOptions(ctypes.Structure):
         ("time_kernels", c_int32),
         ("sweep_all_pairs", c_int32),
         ("verbose", c_int32),
-        ("reserved", c_int32 * 3),
+        ("recompute_abstraction", c_int32),
+        ("reserved", c_int32 * 2),
     ]
 
 

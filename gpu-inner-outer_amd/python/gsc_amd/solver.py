@@ -149,7 +149,7 @@ class Solver:
     """One case resident on one device (gsc_solver_* entry points)."""
 
     def __init__(self, case: Case, device: int = 0, keep_export: bool = False, time_kernels: bool = True,
-                 sweep_all_pairs: bool = False, verbose: bool = False):
+                 sweep_all_pairs: bool = False, verbose: bool = False, recompute_abstraction: bool = False):
         self.lib = capi.load()
         self.case = case
         opts = capi.SolverOptions()
@@ -159,6 +159,7 @@ class Solver:
         opts.time_kernels = int(time_kernels)
         opts.sweep_all_pairs = int(sweep_all_pairs)
         opts.verbose = int(verbose)
+        opts.recompute_abstraction = int(recompute_abstraction)
         d, keep = case.to_desc()
         h = ctypes.c_void_p()
         err = ctypes.create_string_buffer(512)

@@ -250,7 +250,9 @@ typedef struct gsc_solver_options {
     int32_t sweep_all_pairs; /* 1: test every pair every iteration (the reference's work
                                 schedule); 0: test pairs of unsettled states only */
     int32_t verbose;         /* print per-iteration progress like kernels.cu:746-756 */
-    int32_t reserved[3];
+    int32_t recompute_abstraction; /* 1: store no pair records; every iteration recomputes the
+                                      successor boxes it tests in FP64 (same results) */
+    int32_t reserved[2];
 } gsc_solver_options;
 
 typedef struct gsc_solve_summary {

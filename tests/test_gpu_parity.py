@@ -68,6 +68,46 @@ def test_sweep_all_pairs_schedule(gpu, oracle):
     assert summ["pairs_tested"] == c.state_count() * c.input_count() * summ["iterations"]
 
 
+RECOMPUTE_SEEDS = list(range(16))
+
+
+@pytest.mark.parametrize("which", ["builtin:default", "builtin:wraparound", "builtin:smoke", "c1"]
+                         + [f"seed:{s}" for s in RECOMPUTE_SEEDS])
+def test_recompute_abstraction_mode(gpu, oracle, which):
+    # No stored pair records: every tested box is recomputed in FP64 each iteration.
+    if which.startswith("builtin:"):
+        c = cases.builtin(which.split(":")[1])
+        c.max_iterations = min(c.max_iterations, 6)
+    elif which == "c1":
+        c = cases.unicycle_30x30x24x10(sample_time=0.5)
+    else:
+        c = small_random_case(int(which.split(":")[1]))
+        try:
+            oracle.masks(c)
+        except ValueError:
+            pytest.skip("configuration error case")
+    print(f"[case] recompute {which}", flush=True)
+    ref = oracle.run(c, keep_abstraction=False)
+    with Solver(c, recompute_abstraction=True) as s:
+        assert s.build_abstraction() == 0.0
+        s.solve()
+        got = s.download()
+        with pytest.raises(GscError, match="not stored in recompute mode"):
+            s.download(abstraction=True)
+    assert_same_solution(got, ref, f"recompute {which}")
+    assert [st["newly_reachable"] for st in got.iteration_stats] == [int(v) for v in ref["newly_reachable"]]
+
+
+def test_recompute_sweep_all_pairs(gpu, oracle):
+    c = cases.unicycle_30x30x24x10(sample_time=0.5)
+    ref = oracle.run(c, keep_abstraction=False)
+    with Solver(c, recompute_abstraction=True, sweep_all_pairs=True) as s:
+        summ = s.solve()
+        got = s.download()
+    assert_same_solution(got, ref, "recompute sweep")
+    assert summ["pairs_tested"] == c.state_count() * c.input_count() * summ["iterations"]
+
+
 def test_resolve_is_idempotent(gpu, oracle):
     # Re-seeding and re-solving on resident state gives the same answer twice.
     c = cases.unicycle_30x30x24x10(sample_time=0.5)
