@@ -54,3 +54,34 @@ def test_published_case_bit_exact(gpu):
 
 def test_unicycle_91x105x42x10_bit_exact(gpu):
     _check(cases.unicycle_91x105x42x10(), _golden("unicycle_91x105x42x10"), abstraction=True)
+
+
+def test_unicycle_128x128x64x16_certified(gpu, oracle):
+    """1,677,721,600 pairs: too large for a full oracle run in a test, so the
+    result is certified on sampled states against the oracle's own successor
+    boxes and queries (tests/certify.py), and the three device schedules —
+    stored records, recomputed boxes, every pair every iteration — must agree
+    bit for bit."""
+    from certify import certify
+    from casegen import SOLVE_KEYS
+    from gsc_amd.solver import Solver
+
+    case = cases.unicycle_128x128x64x16()
+    print(f"[case] {case.name} stored", flush=True)
+    with Solver(case, time_kernels=False) as s:
+        s.build_abstraction()
+        s.solve()
+        stored = s.download()
+    out = certify(oracle, case, stored, n_chunks=40, chunk=64, seed=1)
+    assert out["checked"] > 1000
+    for label, kw in (("recompute", {"recompute_abstraction": True}), ("sweep", {"sweep_all_pairs": True})):
+        print(f"[case] {case.name} {label}", flush=True)
+        with Solver(case, time_kernels=False, **kw) as s:
+            s.build_abstraction()
+            s.solve()
+            other = s.download()
+        for k in SOLVE_KEYS:
+            assert np.array_equal(getattr(other, k), getattr(stored, k)), f"{label}: {k}"
+        assert other.iterations == stored.iterations and other.result_message == stored.result_message
+        assert [a["newly_reachable"] for a in other.iteration_stats] == \
+            [a["newly_reachable"] for a in stored.iteration_stats]
