@@ -61,6 +61,40 @@ GSC_HD inline uint32_t rec_ext0(uint32_t r) { return (r >> 10) & 31u; }
 GSC_HD inline int rec_off1(uint32_t r) { return static_cast<int32_t>(r << 7) >> 22; }
 GSC_HD inline uint32_t rec_ext1(uint32_t r) { return (r >> 25) & 31u; }
 
+// Record placement. Within a (θ, v) plane (plane base = plane * I_pad * PQ,
+// PQ = cells per plane) the records come in blocks of kRecBlock consecutive
+// inputs: input i of cell q sits at (i / kRecBlock * PQ + q) * kRecBlock +
+// i % kRecBlock. One state's block is one 32-byte sector, loaded with two
+// 16-byte vector loads, and neighbouring cells' blocks are adjacent, so the
+// satisfaction pass fetches whole sectors of the states it tests however
+// sparse they are.
+constexpr uint32_t kRecBlock = 8;
+GSC_HD inline uint32_t rec_offset(uint32_t PQ, uint32_t q, uint32_t i) {
+    return ((i / kRecBlock) * PQ + q) * kRecBlock + i % kRecBlock;
+}
+
+// ---- state hull (4 bytes per state) ----------------------------------------
+// The union of the x / y ranges of a state's valid successor boxes, relative to
+// the state's own cell: int8 dx_lo, dx_hi, dy_lo, dy_hi in bytes 0..3. Together
+// with the per-θ-cell and per-v-cell unions of the θ / v ranges it bounds every
+// box the state can be tested against, so a state whose hull saw no newly
+// reached cell since its last test gets the same (unsatisfied) answer and can
+// be skipped. Two sentinels:
+constexpr uint32_t kHullNone = 0x807F807Fu;  // no valid pair: never satisfiable
+constexpr uint32_t kHullAll = 0x7F807F80u;   // not representable (escaped box): always test
+constexpr int kHullMaxOffset = 100;
+
+// Chunk hull (uint4 x0, x1, y0, y1 in absolute cells) of 64 consecutive cells
+// of a plane: the union of their state hulls, or one of two sentinels.
+constexpr uint32_t kChunkAll = 0xFFFFFFFFu;   // x0 == kChunkAll: some cell always tests
+constexpr uint32_t kChunkNone = 0xFFFFFFFEu;  // x0 == kChunkNone: no cell can be satisfied
+
+GSC_HD inline uint32_t hull_pack(int dxlo, int dxhi, int dylo, int dyhi) {
+    return (static_cast<uint32_t>(dxlo) & 0xFFu) | ((static_cast<uint32_t>(dxhi) & 0xFFu) << 8) |
+           ((static_cast<uint32_t>(dylo) & 0xFFu) << 16) | ((static_cast<uint32_t>(dyhi) & 0xFFu) << 24);
+}
+GSC_HD inline int hull_byte(uint32_t h, int k) { return static_cast<int8_t>((h >> (8 * k)) & 0xFFu); }
+
 // Overflow entry for escaped pairs.
 struct OverflowEntry {
     uint32_t min_flat;
@@ -98,7 +132,14 @@ struct DevCase {
     int32_t band_enabled;
     double band_c, band_w;
     uint32_t n_obstacles, n_targets;
+    int32_t hull_enabled;  // frontier skipping available (wrap along θ or v only)
+    // Coarse prefix tables: the full tables sampled at x, y multiples of
+    // kCoarse (and at n_x, n_y), θ and v at full resolution.
+    uint32_t cshape[2];    // coarse x / y positions: ceil(n / kCoarse) + 1
+    uint32_t cstride[4];
 };
+
+constexpr uint32_t kCoarse = 4;
 
 GSC_HD inline double cell_center(double lb, uint32_t idx, double eta) {
     // grid.hpp:42 / :76

@@ -56,6 +56,15 @@ __device__ inline bool loop_done(const IterCtl* ctl) {
     return __atomic_load_n(&ctl->done, __ATOMIC_RELAXED) != 0;
 }
 
+// `done` and `iterations` are adjacent words of IterCtl: one 64-bit load gives
+// both (whether to run, and which iteration this is).
+__device__ __forceinline__ uint64_t loop_state(const IterCtl* ctl) {
+    static_assert(offsetof(IterCtl, iterations) == offsetof(IterCtl, done) + 4, "done, iterations adjacent");
+    return __atomic_load_n(reinterpret_cast<const unsigned long long*>(&ctl->done), __ATOMIC_RELAXED);
+}
+__device__ __forceinline__ bool state_done(uint64_t st) { return static_cast<uint32_t>(st) != 0; }
+__device__ __forceinline__ uint32_t state_next_iteration(uint64_t st) { return static_cast<uint32_t>(st >> 32) + 1u; }
+
 __device__ inline void unflatten(const DevCase& c, uint32_t s, uint32_t idx[4]) {
     // grid.hpp:30-36 with x fastest.
     uint32_t t = s / c.shape[0];
@@ -182,9 +191,14 @@ __device__ __forceinline__ void prefix_rows_body(const DevCase& c, const uint8_t
 }
 
 __global__ __launch_bounds__(256) void prefix_rows_kernel(DevCase c, const uint8_t* __restrict__ a,
-                                                           const uint8_t* __restrict__ b, uint32_t* __restrict__ P,
+                                                           const uint8_t* __restrict__ b, uint32_t* P0, uint32_t* P1,
                                                            const IterCtl* ctl) {
-    if (ctl && loop_done(ctl)) return;
+    uint32_t* P = P0;
+    if (ctl) {
+        const uint64_t st = loop_state(ctl);
+        if (state_done(st)) return;
+        P = (state_next_iteration(st) & 1u) ? P1 : P0;
+    }
     prefix_rows_body(c, a, b, P, blockIdx.x);
 }
 
@@ -247,14 +261,48 @@ __device__ __forceinline__ void prefix_axis_body(const DevCase& c, int axis, uin
     }
 }
 
-__global__ __launch_bounds__(256) void prefix_axis_kernel(DevCase c, int axis, uint32_t* __restrict__ P,
+__global__ __launch_bounds__(256) void prefix_axis_kernel(DevCase c, int axis, uint32_t* P0, uint32_t* P1,
                                                            const IterCtl* ctl) {
-    if (ctl && loop_done(ctl)) return;
+    uint32_t* P = P0;
+    if (ctl) {
+        const uint64_t st = loop_state(ctl);
+        if (state_done(st)) return;
+        P = (state_next_iteration(st) & 1u) ? P1 : P0;
+    }
     prefix_axis_body(c, axis, P, blockIdx.x);
 }
 
+// Coarse sample: C[X, Y, z, w] = P[min(kCoarse X, n_x), min(kCoarse Y, n_y), z, w]
+// over the padded θ / v indices. Thread per coarse entry.
+__device__ __forceinline__ void coarse_body(const DevCase& c, const uint32_t* __restrict__ P, uint32_t* __restrict__ C,
+                                            uint32_t t) {
+    const uint32_t cx = c.cshape[0], cy = c.cshape[1];
+    const uint32_t n = cx * cy * (c.shape[2] + 1) * (c.shape[3] + 1);
+    if (t >= n) return;
+    const uint32_t X = t % cx;
+    uint32_t r = t / cx;
+    const uint32_t Y = r % cy;
+    r /= cy;
+    const uint32_t z = r % (c.shape[2] + 1);
+    const uint32_t w = r / (c.shape[2] + 1);
+    const uint32_t x = min(X * kCoarse, c.shape[0]);
+    const uint32_t y = min(Y * kCoarse, c.shape[1]);
+    C[t] = P[x + y * c.pstride[1] + z * c.pstride[2] + w * c.pstride[3]];
+}
+
+__global__ __launch_bounds__(256) void coarse_kernel(DevCase c, const uint32_t* P0, const uint32_t* P1, uint32_t* C0,
+                                                      uint32_t* C1, const IterCtl* ctl) {
+    bool odd = false;
+    if (ctl) {
+        const uint64_t st = loop_state(ctl);
+        if (state_done(st)) return;
+        odd = (state_next_iteration(st) & 1u) != 0;
+    }
+    coarse_body(c, odd ? P1 : P0, odd ? C1 : C0, blockIdx.x * 256u + threadIdx.x);
+}
+
 // ---------------------------------------------------------------------------
-// Pair records are stored plane-major: rec[(plane * I + input) * PQ + q] with
+// Pair records are stored plane-major, in 8-input blocks (rec_offset) with
 // plane = θ cell + n_θ * v cell and q = x + n_x * y (PQ = n_x * n_y cells per
 // plane). A wave owns 64 consecutive cells q of one plane and walks the
 // inputs in order, so
@@ -402,9 +450,13 @@ __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
     const double rad0 = A.rad[2 * iv];
     const double rad1 = A.rad[2 * iv + 1];
     const int wd = c.wrap_dim;
-    const uint64_t base = static_cast<uint64_t>(plane) * c.I_pad * c.PQ + q;
-    if (live)
-        for (uint32_t i = c.I; i < c.I_pad; ++i) A.rec[base + static_cast<uint64_t>(i) * c.PQ] = kRecInvalid;
+    uint32_t* const rec_plane = A.rec + static_cast<uint64_t>(plane) * c.I_pad * c.PQ;
+    const bool store = A.rec != nullptr;  // false: state hulls only
+    if (live && store)
+        for (uint32_t i = c.I; i < c.I_pad; ++i) rec_plane[rec_offset(c.PQ, q, i)] = kRecInvalid;
+    // Running hull of the state's valid boxes (see gsc_device.hpp).
+    int hx0 = kHullMaxOffset + 1, hx1 = -kHullMaxOffset - 1, hy0 = kHullMaxOffset + 1, hy1 = -kHullMaxOffset - 1;
+    bool any_valid = false, unrepresentable = false;
     uint32_t i = 0;
     for (uint32_t iw = 0; iw < c.n_w; ++iw) {
         const uint2 b2 = A.box2[iz * c.n_w + iw];
@@ -423,7 +475,18 @@ __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
                 const int off1 = static_cast<int>(lo[1]) - static_cast<int>(iy);
                 const bool fits = regular && off0 >= kOffMin && off0 <= kOffMax && off1 >= kOffMin &&
                                   off1 <= kOffMax && hi[0] - lo[0] <= kExtMax && hi[1] - lo[1] <= kExtMax;
+                any_valid = true;
                 if (fits) {
+                    hx0 = min(hx0, off0);
+                    hx1 = max(hx1, off0 + static_cast<int>(hi[0] - lo[0]));
+                    hy0 = min(hy0, off1);
+                    hy1 = max(hy1, off1 + static_cast<int>(hi[1] - lo[1]));
+                } else {
+                    unrepresentable = true;
+                }
+                if (!store) {
+                    // hulls only
+                } else if (fits) {
                     r = rec_pack(off0, hi[0] - lo[0], off1, hi[1] - lo[1], false);
                 } else {
                     const int64_t vc = box_count_i64(A.prefix_valid, c.pstride, wd, c.shape[wd], lo, hi);
@@ -441,8 +504,49 @@ __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
                     }
                 }
             }
-            if (live) A.rec[base + static_cast<uint64_t>(i) * c.PQ] = r;
+            if (live && store) rec_plane[rec_offset(c.PQ, q, i)] = r;
         }
+    }
+    if (live && A.hull) {
+        uint32_t h;
+        if (!any_valid)
+            h = kHullNone;
+        else if (unrepresentable || hx0 < -kHullMaxOffset || hx1 > kHullMaxOffset || hy0 < -kHullMaxOffset ||
+                 hy1 > kHullMaxOffset)
+            h = kHullAll;
+        else
+            h = hull_pack(hx0, hx1, hy0, hy1);
+        A.hull[static_cast<size_t>(plane) * c.PQ + q] = h;
+    }
+    if (A.chunk_hull) {
+        // Union over the wave's 64 cells (absolute coordinates).
+        uint32_t h = kHullNone;
+        if (live) {
+            if (!any_valid)
+                h = kHullNone;
+            else if (unrepresentable || hx0 < -kHullMaxOffset || hx1 > kHullMaxOffset || hy0 < -kHullMaxOffset ||
+                     hy1 > kHullMaxOffset)
+                h = kHullAll;
+            else
+                h = hull_pack(hx0, hx1, hy0, hy1);
+        }
+        const bool all = __ballot(h == kHullAll) != 0;
+        const bool none = __ballot(h != kHullNone) == 0;
+        uint32_t x0 = 0xFFFFFFFFu, x1 = 0, y0 = 0xFFFFFFFFu, y1 = 0;
+        if (h != kHullNone && h != kHullAll) {
+            x0 = ix + hull_byte(h, 0);
+            x1 = ix + hull_byte(h, 1);
+            y0 = iy + hull_byte(h, 2);
+            y1 = iy + hull_byte(h, 3);
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            x0 = min(x0, static_cast<uint32_t>(__shfl_xor(x0, off, 64)));
+            x1 = max(x1, static_cast<uint32_t>(__shfl_xor(x1, off, 64)));
+            y0 = min(y0, static_cast<uint32_t>(__shfl_xor(y0, off, 64)));
+            y1 = max(y1, static_cast<uint32_t>(__shfl_xor(y1, off, 64)));
+        }
+        if (lane == 0)
+            A.chunk_hull[task] = all ? uint4{kChunkAll, 0, 0, 0} : none ? uint4{kChunkNone, 0, 0, 0} : uint4{x0, x1, y0, y1};
     }
 }
 
@@ -460,7 +564,7 @@ __global__ __launch_bounds__(256) void export_kernel(DevCase c, const uint32_t* 
     const uint32_t i = static_cast<uint32_t>(p - static_cast<uint64_t>(s) * c.I);
     const uint32_t plane = s / c.PQ;
     const uint32_t q = s - plane * c.PQ;
-    const uint32_t r = rec[(static_cast<uint64_t>(plane) * c.I_pad + i) * c.PQ + q];
+    const uint32_t r = rec[static_cast<uint64_t>(plane) * c.I_pad * c.PQ + rec_offset(c.PQ, q, i)];
     if (r == kRecInvalid) {
         out_min[o] = 0;
         out_max[o] = 0;
@@ -494,8 +598,8 @@ __global__ __launch_bounds__(256) void export_kernel(DevCase c, const uint32_t* 
 // sat = !dead && count(unreached valid cells in the box) == 0, which equals the
 // reference's `valid && cntV > 0 && cntR == cntV` because reachable ⊆ valid
 // makes cntV - cntR the count of valid, unreached cells.
-__device__ inline bool count_test(const SatisfyArgs& A, uint32_t r, uint32_t ix, uint32_t iy, uint2 b2, uint2 b3,
-                                        uint32_t* rec_slot) {
+__device__ inline bool count_test(const SatisfyArgs& A, const uint32_t* PU, uint32_t r, uint32_t ix, uint32_t iy,
+                                  uint2 b2, uint2 b3, uint32_t* rec_slot) {
     const DevCase& c = A.c;
     const int wd = c.wrap_dim;
     if (r & kRecEscape) {
@@ -504,7 +608,7 @@ __device__ inline bool count_test(const SatisfyArgs& A, uint32_t r, uint32_t ix,
         uint32_t lo[4], hi[4];
         unflatten(c, o.min_flat, lo);
         unflatten(c, o.max_flat, hi);
-        return box_count_i64(A.prefix_unreached, c.pstride, wd, c.shape[wd], lo, hi) == 0;
+        return box_count_i64(PU, c.pstride, wd, c.shape[wd], lo, hi) == 0;
     }
     uint32_t lo[4], hi[4];
     lo[0] = ix + rec_off0(r);
@@ -515,7 +619,7 @@ __device__ inline bool count_test(const SatisfyArgs& A, uint32_t r, uint32_t ix,
     hi[2] = b2.y;
     lo[3] = b3.x;
     hi[3] = b3.y;
-    if (box_count_wrap_u32(A.prefix_unreached, c.pstride, wd, c.shape[wd], lo, hi) != 0) return false;
+    if (box_count_wrap_u32(PU, c.pstride, wd, c.shape[wd], lo, hi) != 0) return false;
     // No unreached valid cell: satisfied unless the box has no valid cell at all;
     // that answer never changes, so it is memoised in the record.
     if (box_count_wrap_u32(A.prefix_valid, c.pstride, wd, c.shape[wd], lo, hi) == 0) {
@@ -573,14 +677,67 @@ __device__ __forceinline__ void finish_body(IterCtl* ctl, IterRecord* records, u
     __atomic_store_n(&ctl->done, done, __ATOMIC_RELAXED);
 }
 
-// `done` and `iterations` are adjacent words of IterCtl: one 64-bit load gives
-// both (whether to run, and which iteration this is).
-__device__ __forceinline__ uint64_t loop_state(const IterCtl* ctl) {
-    static_assert(offsetof(IterCtl, iterations) == offsetof(IterCtl, done) + 4, "done, iterations adjacent");
-    return __atomic_load_n(reinterpret_cast<const unsigned long long*>(&ctl->done), __ATOMIC_RELAXED);
+
+// Whether a cell inside the hull of state (ix, iy, plane) turned from unreached
+// to reached between the tables of iterations k - 1 (Pprev) and k (Pcur). The
+// unreached set only shrinks, so the two counts differ exactly then.
+// Two levels: the box snapped outward to kCoarse-cell x / y tiles on the small
+// coarse tables (cache-resident) first, the exact box on the full tables only
+// when the coarse box saw a change.
+__device__ __forceinline__ bool box_changed(const SatisfyArgs& A, const uint32_t lo[4], const uint32_t hi[4],
+                                            const uint32_t* Ccur, const uint32_t* Cprev, const uint32_t* Pcur,
+                                            const uint32_t* Pprev, bool exact) {
+    const DevCase& c = A.c;
+    const int wd = c.wrap_dim;
+    uint32_t clo[4] = {lo[0] / kCoarse, lo[1] / kCoarse, lo[2], lo[3]};
+    uint32_t chi[4] = {(hi[0] + kCoarse) / kCoarse - 1, (hi[1] + kCoarse) / kCoarse - 1, hi[2], hi[3]};
+    if (box_count_wrap_u32(Cprev, c.cstride, wd, c.shape[wd], clo, chi) ==
+        box_count_wrap_u32(Ccur, c.cstride, wd, c.shape[wd], clo, chi))
+        return false;
+    if (!exact) return true;
+    return box_count_wrap_u32(Pprev, c.pstride, wd, c.shape[wd], lo, hi) !=
+           box_count_wrap_u32(Pcur, c.pstride, wd, c.shape[wd], lo, hi);
 }
-__device__ __forceinline__ bool state_done(uint64_t st) { return static_cast<uint32_t>(st) != 0; }
-__device__ __forceinline__ uint32_t state_next_iteration(uint64_t st) { return static_cast<uint32_t>(st >> 32) + 1u; }
+
+__device__ __forceinline__ bool hull_changed(const SatisfyArgs& A, uint32_t h, uint32_t ix, uint32_t iy,
+                                             uint32_t plane, const uint32_t* Ccur, const uint32_t* Cprev,
+                                             const uint32_t* Pcur, const uint32_t* Pprev) {
+    if (h == kHullAll) return true;
+    if (h == kHullNone) return false;
+    const DevCase& c = A.c;
+    uint32_t lo[4], hi[4];
+    lo[0] = ix + hull_byte(h, 0);
+    hi[0] = ix + hull_byte(h, 1);
+    lo[1] = iy + hull_byte(h, 2);
+    hi[1] = iy + hull_byte(h, 3);
+    const uint2 hz = A.hull2[plane % c.shape[2]];
+    const uint2 hv = A.hull3[plane / c.shape[2]];
+    lo[2] = hz.x;
+    hi[2] = hz.y;
+    lo[3] = hv.x;
+    hi[3] = hv.y;
+    return box_changed(A, lo, hi, Ccur, Cprev, Pcur, Pprev, true);
+}
+
+// The same test for a 64-cell chunk of a plane (uniform across the wave).
+__device__ __forceinline__ bool chunk_changed(const SatisfyArgs& A, uint4 ch, uint32_t plane, const uint32_t* Ccur,
+                                              const uint32_t* Cprev) {
+    if (ch.x == kChunkAll) return true;
+    if (ch.x == kChunkNone) return false;
+    const DevCase& c = A.c;
+    uint32_t lo[4], hi[4];
+    lo[0] = ch.x;
+    hi[0] = ch.y;
+    lo[1] = ch.z;
+    hi[1] = ch.w;
+    const uint2 hz = A.hull2[plane % c.shape[2]];
+    const uint2 hv = A.hull3[plane / c.shape[2]];
+    lo[2] = hz.x;
+    hi[2] = hz.y;
+    lo[3] = hv.x;
+    hi[3] = hv.y;
+    return box_changed(A, lo, hi, Ccur, Cprev, nullptr, nullptr, false);
+}
 
 // ---------------------------------------------------------------------------
 // One instance per launch.
@@ -633,6 +790,15 @@ __global__ __launch_bounds__(256) void satisfy_recompute_kernel(RecomputeArgs R)
     const uint32_t gpp = (PQ + 255u) / 256u;
     const uint32_t plane = blockIdx.x / gpp;
     const uint32_t g0 = (blockIdx.x - plane * gpp) * 256u;
+    const uint32_t* const PU = A.prefix_unreached[iter & 1u];
+    // The first iteration compares with "nothing reached yet" (the valid table).
+    const uint32_t* const Pprev = iter == 1 ? A.prefix_valid : A.prefix_unreached[(iter - 1u) & 1u];
+    const uint32_t* const CU = A.coarse_unreached[iter & 1u];
+    const uint32_t* const Cprev = iter == 1 ? A.coarse_valid : A.coarse_unreached[(iter - 1u) & 1u];
+    const bool skip_unchanged = !A.sweep_all && c.hull_enabled;
+    bool chunk_live = true;
+    if (skip_unchanged && g0 + 64u * wave < PQ)
+        chunk_live = chunk_changed(A, A.chunk_hull[plane * c.chunks_per_plane + (g0 >> 6) + wave], plane, CU, Cprev);
     {
         const uint32_t q = g0 + threadIdx.x;
         const uint32_t s = plane * PQ + q;
@@ -647,6 +813,10 @@ __global__ __launch_bounds__(256) void satisfy_recompute_kernel(RecomputeArgs R)
                 f = (use ? kUse : 0) | (is_cand ? kCand : 0) | (reached ? kReached : 0) | (certified ? kCertified : 0);
             }
             eval = A.sweep_all ? true : (f & kUse) != 0;
+            if (eval && skip_unchanged) {
+                const uint32_t iy0 = q / c.shape[0];
+                eval = chunk_live && hull_changed(A, A.hull[s], q - iy0 * c.shape[0], iy0, plane, CU, Cprev, PU, Pprev);
+            }
         }
         const uint64_t m = __ballot(eval);
         if (lane == 0) s_wave_n[wave] = static_cast<uint32_t>(__popcll(m));
@@ -707,12 +877,12 @@ __global__ __launch_bounds__(256) void satisfy_recompute_kernel(RecomputeArgs R)
                         if ((window & ((2ull << (hi[0] - lo[0])) - 1ull)) != 0) continue;
                     }
                     ++counted;
-                    sat = box_count_wrap_u32(A.prefix_unreached, c.pstride, wd, c.shape[wd], lo, hi) == 0 &&
+                    sat = box_count_wrap_u32(PU, c.pstride, wd, c.shape[wd], lo, hi) == 0 &&
                           box_count_wrap_u32(A.prefix_valid, c.pstride, wd, c.shape[wd], lo, hi) != 0;
                 } else {
                     ++counted;
                     sat = box_count_i64(A.prefix_valid, c.pstride, wd, c.shape[wd], lo, hi) != 0 &&
-                          box_count_i64(A.prefix_unreached, c.pstride, wd, c.shape[wd], lo, hi) == 0;
+                          box_count_i64(PU, c.pstride, wd, c.shape[wd], lo, hi) == 0;
                 }
                 if (sat) {
                     if (found == GSC_INVALID_INPUT) found = i;
@@ -795,8 +965,10 @@ __global__ __launch_bounds__(256) void prefix_rows_batch_kernel(const BatchInsta
                                                                  const uint32_t* __restrict__ starts, uint32_t n) {
     const uint32_t k = find_instance(starts, n, blockIdx.x);
     const BatchInstance& I = B[k];
-    if (loop_done(I.sat.ctl)) return;
-    prefix_rows_body(I.sat.c, I.sat.valid, I.sat.reach, I.prefix_unreached, blockIdx.x - starts[k]);
+    const uint64_t st = loop_state(I.sat.ctl);
+    if (state_done(st)) return;
+    prefix_rows_body(I.sat.c, I.sat.valid, I.sat.reach, I.prefix_unreached[state_next_iteration(st) & 1u],
+                     blockIdx.x - starts[k]);
 }
 
 __global__ __launch_bounds__(256) void prefix_axis_batch_kernel(const BatchInstance* __restrict__ B,
@@ -804,8 +976,9 @@ __global__ __launch_bounds__(256) void prefix_axis_batch_kernel(const BatchInsta
                                                                  int axis) {
     const uint32_t k = find_instance(starts, n, blockIdx.x);
     const BatchInstance& I = B[k];
-    if (loop_done(I.sat.ctl)) return;
-    prefix_axis_body(I.sat.c, axis, I.prefix_unreached, blockIdx.x - starts[k]);
+    const uint64_t st = loop_state(I.sat.ctl);
+    if (state_done(st)) return;
+    prefix_axis_body(I.sat.c, axis, I.prefix_unreached[state_next_iteration(st) & 1u], blockIdx.x - starts[k]);
 }
 
 __global__ __launch_bounds__(256) void unreached_bits_batch_kernel(const BatchInstance* __restrict__ B,
@@ -814,6 +987,17 @@ __global__ __launch_bounds__(256) void unreached_bits_batch_kernel(const BatchIn
     const BatchInstance& I = B[k];
     if (loop_done(I.sat.ctl)) return;
     unreached_bits_body(I.sat.c.S, I.sat.valid, I.sat.reach, I.unreached_bits, blockIdx.x - starts[k]);
+}
+
+__global__ __launch_bounds__(256) void coarse_batch_kernel(const BatchInstance* __restrict__ B,
+                                                            const uint32_t* __restrict__ starts, uint32_t n) {
+    const uint32_t k = find_instance(starts, n, blockIdx.x);
+    const BatchInstance& I = B[k];
+    const uint64_t st = loop_state(I.sat.ctl);
+    if (state_done(st)) return;
+    const uint32_t par = state_next_iteration(st) & 1u;
+    coarse_body(I.sat.c, I.prefix_unreached[par], I.coarse_unreached[par],
+                (blockIdx.x - starts[k]) * 256u + threadIdx.x);
 }
 
 // The batched kernel reads its case's arguments from global memory; there the

@@ -60,16 +60,27 @@ struct AbstractionArgs {
     const double* rad;           // [n_v][2] x / y box radius
     const uint32_t* fail2d;      // (n0+1)x(n1+1) prefix of constraint failures, or null
     const uint32_t* prefix_valid;
-    uint32_t* rec;
+    uint32_t* rec;                // null: compute the state hulls only (recompute mode)
     OverflowEntry* overflow;
     uint32_t* overflow_count;
     uint32_t overflow_capacity;
+    uint32_t* hull;               // [S] state hulls
+    uint4* chunk_hull;            // [n_tasks] union of the hulls of each 64-cell chunk
 };
 
 struct SatisfyArgs {
     DevCase c;
-    const uint32_t* prefix_unreached;
+    // Prefix tables of valid-and-unreached states, double-buffered by iteration
+    // parity: iteration k reads table k & 1 and compares it with table (k-1) & 1,
+    // built one iteration earlier, to find states whose hull saw no change.
+    const uint32_t* prefix_unreached[2];
     const uint32_t* prefix_valid;
+    const uint32_t* coarse_unreached[2];  // coarse samples of prefix_unreached[0/1]
+    const uint32_t* coarse_valid;         // coarse sample of prefix_valid
+    const uint32_t* hull;   // [S] state hulls
+    const uint4* chunk_hull;  // [planes * chunks_per_plane] absolute x0, x1, y0, y1 of 64 cells
+    const uint2* hull2;     // [n_θ] union of the θ ranges over ω (lo > hi: wraps)
+    const uint2* hull3;     // [n_v] union of the v ranges over a
     const unsigned long long* unreached_bits;
     const uint32_t* probe_zw;  // [plane][I_pad]: lo2*stride2 + lo3*stride3 of the pair's box
     uint32_t* rec;
@@ -98,7 +109,8 @@ struct RecomputeArgs {
 // arguments plus what the prefix, bitmask, seed and finish stages write.
 struct BatchInstance {
     SatisfyArgs sat;
-    uint32_t* prefix_unreached;
+    uint32_t* prefix_unreached[2];
+    uint32_t* coarse_unreached[2];
     unsigned long long* unreached_bits;
     IterRecord* records;
     unsigned long long candidate_count;
@@ -119,9 +131,16 @@ __global__ void masks_kernel(DevCase c, const gsc_rect4* obstacles, const gsc_re
 __global__ void seed_kernel(uint32_t S, const uint8_t* cand, uint8_t* reach, uint32_t* ctrl, uint32_t* rstep);
 __global__ void reset_ctl_kernel(IterCtl* ctl, unsigned long long candidate_count);
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t value);
-__global__ void prefix_rows_kernel(DevCase c, const uint8_t* a, const uint8_t* b, uint32_t* P, const IterCtl* ctl);
-__global__ void prefix_axis_kernel(DevCase c, int axis, uint32_t* P, const IterCtl* ctl);
+// With ctl: the table of the coming iteration k = ctl->iterations + 1 is P[k & 1];
+// without: P0.
+__global__ void prefix_rows_kernel(DevCase c, const uint8_t* a, const uint8_t* b, uint32_t* P0, uint32_t* P1,
+                                   const IterCtl* ctl);
+__global__ void prefix_axis_kernel(DevCase c, int axis, uint32_t* P0, uint32_t* P1, const IterCtl* ctl);
 __global__ void increments_kernel(DevCase c, const double2* cs, const double* vt, double2* inc);
+// Coarse sample of a prefix table (with ctl: of table k & 1 into coarse k & 1).
+__global__ void coarse_kernel(DevCase c, const uint32_t* P0, const uint32_t* P1, uint32_t* C0, uint32_t* C1,
+                              const IterCtl* ctl);
+__global__ void coarse_batch_kernel(const BatchInstance* B, const uint32_t* starts, uint32_t n);
 __global__ void unreached_bits_kernel(uint32_t S, const uint8_t* valid, const uint8_t* reach, unsigned long long* bits,
                                       const IterCtl* ctl);
 __global__ void probe_zw_kernel(DevCase c, const uint2* box2, const uint2* box3, uint32_t* probe_zw);

@@ -20,7 +20,7 @@
 namespace gsc_amd {
 namespace {
 
-enum Stage : uint32_t { kSeed = 0, kRows, kAxis1, kAxis2, kAxis3, kBits, kSat, kStages };
+enum Stage : uint32_t { kSeed = 0, kRows, kAxis1, kAxis2, kAxis3, kCoarseStage, kBits, kSat, kStages };
 
 using Clock = std::chrono::steady_clock;
 
@@ -154,6 +154,7 @@ void gsc_server::plan() {
             const int o2 = axis == 3 ? 2 : 3;
             blocks[kAxis1 + axis - 1] = (static_cast<uint64_t>(c.shape[0]) * c.shape[o1] * c.shape[o2] + 255) / 256;
         }
+        blocks[kCoarseStage] = (s.coarse_count + 255) / 256;
         blocks[kBits] = (c.S + 255) / 256;
         blocks[kSat] = satisfy_blocks(c, static_cast<uint32_t>(opts.tile_wg));
         for (uint32_t st = 0; st < kStages; ++st) {
@@ -163,7 +164,15 @@ void gsc_server::plan() {
         }
         BatchInstance& b = h_inst[p];
         b.sat.c = c;
-        b.sat.prefix_unreached = s.prefix_unreached;
+        b.sat.prefix_unreached[0] = s.prefix_unreached[0];
+        b.sat.prefix_unreached[1] = s.prefix_unreached[1];
+        b.sat.coarse_unreached[0] = s.coarse_unreached[0];
+        b.sat.coarse_unreached[1] = s.coarse_unreached[1];
+        b.sat.coarse_valid = s.coarse_valid;
+        b.sat.hull = s.hull;
+        b.sat.chunk_hull = s.chunk_hull;
+        b.sat.hull2 = s.hull2;
+        b.sat.hull3 = s.hull3;
         b.sat.prefix_valid = s.prefix_valid;
         b.sat.unreached_bits = s.ubits;
         b.sat.probe_zw = s.probe_zw;
@@ -178,7 +187,10 @@ void gsc_server::plan() {
         b.sat.rstep = s.rstep;
         b.sat.ctl = s.ctl;
         b.sat.sweep_all = 0;
-        b.prefix_unreached = s.prefix_unreached;
+        b.prefix_unreached[0] = s.prefix_unreached[0];
+        b.prefix_unreached[1] = s.prefix_unreached[1];
+        b.coarse_unreached[0] = s.coarse_unreached[0];
+        b.coarse_unreached[1] = s.coarse_unreached[1];
         b.unreached_bits = s.ubits;
         b.records = d_records + static_cast<size_t>(active[p]->slot) * kRing;
         b.candidate_count = s.candidate_count;
@@ -208,6 +220,8 @@ void gsc_server::enqueue_step() {
             hipLaunchKernelGGL(prefix_axis_batch_kernel, dim3(grid[s]), dim3(256), 0, iter_stream, d_inst,
                                st + s * (N + 1), N, axis);
         }
+        hipLaunchKernelGGL(coarse_batch_kernel, dim3(grid[kCoarseStage]), dim3(256), 0, iter_stream, d_inst,
+                           st + kCoarseStage * (N + 1), N);
         hipLaunchKernelGGL(unreached_bits_batch_kernel, dim3(grid[kBits]), dim3(256), 0, iter_stream, d_inst,
                            st + kBits * (N + 1), N);
         GSC_HIP(launch_satisfy_batch(d_inst, st + kSat * (N + 1), N, grid[kSat], static_cast<uint32_t>(opts.tile_wg),

