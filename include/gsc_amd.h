@@ -306,8 +306,8 @@ int gsc_config_builtin(int which, gsc_config** out);
  * Batched solves. No reference counterpart: the reference solves one case per
  * run_case_cuda_u32 call (kernels.cu:492-875). A batch holds many independent
  * cases of uneven grid and input sizes resident on one device and advances all
- * of them together: every stage of an iteration (prefix build, bitmask,
- * satisfaction + reduction, stop test) is one launch over the concatenated
+ * of them together: every stage of an iteration (prefix build, coarse sample,
+ * bitmask, satisfaction + reduction, stop test) is one launch over the concatenated
  * work of all unfinished cases, and `iterations_per_launch` such iterations
  * are captured once in a HIP graph that is replayed until every case has
  * stopped. Each case stops on its own rule and keeps its own results, which are
