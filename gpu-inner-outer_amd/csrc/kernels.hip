@@ -759,10 +759,11 @@ __device__ __forceinline__ bool chunk_changed(const SatisfyArgs& A, uint4 ch, ui
 // States are updated in place: the prefix snapshot was built before this
 // launch and only this workgroup touches these states (Jacobi, as kernels.cu:742).
 // The body is satisfy_body.inc.
-// Eight waves per SIMD (64 VGPRs, a few bytes of spill) measured faster than
-// seven without spills: the pass is bound by memory latency, not issue.
+// Seven waves per SIMD (72 VGPRs, no spill) measured faster than eight with
+// spills and than the unbounded 82 VGPRs: the pass waits on memory, so
+// occupancy pays until spill traffic eats it.
 template <uint32_t WG, uint32_t IB>
-__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void satisfy_kernel(SatisfyArgs A) {
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) void satisfy_kernel(SatisfyArgs A) {
     const uint64_t st = loop_state(A.ctl);
     if (state_done(st)) return;
     const uint32_t block = blockIdx.x;
