@@ -107,6 +107,7 @@ struct OverflowEntry {
 struct DevCase {
     double lb[4], ub[4], eta[4];
     double lb_minus_eps[4];  // lb[d] - kEps   (grid.hpp:193)
+    double inv_eta[4];       // 1 / eta[d], for the filtered cell rounding only
     double ub_plus_eps[4];   // ub[d] + kEps
     double wrap_span_minus_eps;  // (ub[w] - lb[w]) - kEps (grid.hpp:198-207)
     uint32_t shape[4];
@@ -212,6 +213,56 @@ GSC_HD inline bool dim_to_cells(const DevCase& c, int d, double mn, double mx, u
     *lo = static_cast<uint32_t>(lower);
     *hi = static_cast<uint32_t>(upper);
     return true;
+}
+
+// ---- filtered rounding of the nominal x / y coordinate ----------------------
+// The reference adds the K trajectory increments to the cell centre one at a
+// time, rounding after every step (x <- x + inc_k, model.hpp:40-41). The sum S
+// of the increments is the same for every cell of a (θ, v) plane and input, so
+// p = x0 + S approximates the reference's coordinate: with A = sum |inc_k| and
+// u = 2^-53, K roundings of partial sums bounded by |x0| + A on each side plus
+// the final addition keep them within 2u(K + 1)(|x0| + A) (the caller's `err`).
+// dim_to_cells_fast carries that bound through the box edges (p -/+ rad), the
+// domain test against lb - kEps / ub + kEps and (edge - lb) / eta, widens it by
+// kFilterWiden, and returns the reference's answer only when every threshold —
+// each floor, each ceil, each domain comparison — is clear of it; otherwise
+// kCellsUndecided, and the caller runs the exact step-by-step sum. Non-wrap
+// dimensions only.
+constexpr double kUnitRoundoff = 1.1102230246251565e-16;  // 2^-53
+constexpr double kFilterWiden = 1024.0;
+enum : int { kCellsUndecided = 0, kCellsReject = 1, kCellsOk = 2 };
+
+GSC_HD inline double filter_error_bound(uint32_t K, double x0, double abs_sum) {
+    return 2.0 * kUnitRoundoff * (static_cast<double>(K) + 1.0) * (fabs(x0) + abs_sum);
+}
+
+GSC_HD inline int dim_to_cells_fast(const DevCase& c, int d, double p, double err, double rad, uint32_t* lo,
+                                    uint32_t* hi) {
+    const double mn = p - rad, mx = p + rad;
+    // Edges: the coordinate's error plus one rounding on each side.
+    const double e_raw = err + 4.0 * kUnitRoundoff * (fabs(p) + fabs(rad));
+    const double e = kFilterWiden * e_raw;
+    if (mn < c.lb_minus_eps[d] - e || mx > c.ub_plus_eps[d] + e) return kCellsReject;
+    if (mn <= c.lb_minus_eps[d] + e || mx >= c.ub_plus_eps[d] - e) return kCellsUndecided;
+    // Quotients: the edge error over eta, plus the roundings of the subtraction,
+    // of the reference's division and of the reciprocal and product here.
+    const double ql = (mn - c.lb[d]) * c.inv_eta[d];
+    const double qh = (mx - c.lb[d]) * c.inv_eta[d];
+    const double eq = kFilterWiden * ((e_raw + 4.0 * kUnitRoundoff * (fabs(mn) + fabs(mx) + fabs(c.lb[d]))) *
+                                          fabs(c.inv_eta[d]) +
+                                      8.0 * kUnitRoundoff * (1.0 + fabs(ql) + fabs(qh)));
+    const double fl = floor(ql), ce = ceil(qh);
+    if (!(ql - eq > fl && ql + eq < fl + 1.0 && qh - eq > ce - 1.0 && qh + eq < ce)) return kCellsUndecided;
+    int64_t lower = static_cast<int64_t>(fl);
+    int64_t upper = static_cast<int64_t>(ce) - 1;
+    const int64_t n = static_cast<int64_t>(c.shape[d]);
+    if (lower < 0) lower = 0;
+    if (upper < 0) upper = 0;
+    if (lower >= n) lower = n - 1;
+    if (upper >= n) upper = n - 1;
+    *lo = static_cast<uint32_t>(lower);
+    *hi = static_cast<uint32_t>(upper);
+    return kCellsOk;
 }
 
 // 16-corner inclusion–exclusion on a padded u32 prefix table (prefix_sum.hpp:65-85)

@@ -340,6 +340,28 @@ __global__ __launch_bounds__(256) void increments_kernel(DevCase c, const double
     inc[t] = double2{c.dt * dx, c.dt * dy};
 }
 
+// Per (plane, input): the sum of the K increments in step order and the sum of
+// their magnitudes, for x and y — the filtered fast path of pair_box
+// (dim_to_cells_fast). Thread per (plane, input).
+__global__ __launch_bounds__(256) void increment_sums_kernel(DevCase c, const double2* __restrict__ inc,
+                                                              double* __restrict__ sums) {
+    const uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<uint64_t>(c.shape[2]) * c.shape[3] * c.I) return;
+    const double2* row = inc + t * c.K;
+    double sx = 0.0, sy = 0.0, ax = 0.0, ay = 0.0;
+    for (uint32_t k = 0; k < c.K; ++k) {
+        const double2 v = row[k];
+        sx = sx + v.x;
+        sy = sy + v.y;
+        ax = ax + fabs(v.x);
+        ay = ay + fabs(v.y);
+    }
+    sums[4 * t] = sx;
+    sums[4 * t + 1] = sy;
+    sums[4 * t + 2] = ax;
+    sums[4 * t + 3] = ay;
+}
+
 // θ/v part of the probe cell per (plane, input): lo2 * stride2 + lo3 * stride3 of
 // the pair's successor box (0 for padding inputs and rejected ranges, whose
 // records are invalid and never probed).
@@ -369,37 +391,49 @@ __global__ __launch_bounds__(256) void probe_zw_kernel(DevCase c, const uint2* _
 //   constraint scan  abstraction.hpp:110-159 as a 2D prefix query over x,y fails
 // Returns false for an invalid pair (abstraction.hpp:187-205). `inc` points at
 // the pair's (plane, input) row of K increments and must be wave-uniform.
-__device__ __forceinline__ bool pair_box(const DevCase& c, ConstPtr<double> inc, uint2 b2, uint2 b3, double x0,
-                                         double y0, double rad0, double rad1, const uint32_t* __restrict__ fail2d,
-                                         uint32_t lo[4], uint32_t hi[4]) {
+__device__ __forceinline__ bool pair_box(const DevCase& c, ConstPtr<double> inc, ConstPtr<double> sums, uint2 b2,
+                                         uint2 b3, double x0, double y0, double rad0, double rad1,
+                                         const uint32_t* __restrict__ fail2d, uint32_t lo[4], uint32_t hi[4]) {
     if (b2.x == kBoxReject || b3.x == kBoxReject) return false;
     lo[2] = b2.x;
     hi[2] = b2.y;
     lo[3] = b3.x;
     hi[3] = b3.y;
-    // Scalar loads, eight steps per wait; the additions stay strictly in step order.
-    double px = x0, py = y0;
-    uint32_t k = 0;
-    for (; k + 8 <= c.K; k += 8) {
-        double tx[8], ty[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            tx[j] = inc[2 * (k + j)];
-            ty[j] = inc[2 * (k + j) + 1];
-        }
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            px = px + tx[j];
-            py = py + ty[j];
-        }
+    // Fast path: x0 + (sum of the increments), decided by dim_to_cells_fast when
+    // its error bound leaves no doubt about the reference's cells (x and y are
+    // plain intervals when the wrap axis is θ or v).
+    int rx = kCellsUndecided, ry = kCellsUndecided;
+    if (c.wrap_dim >= 2) {
+        rx = dim_to_cells_fast(c, 0, x0 + sums[0], filter_error_bound(c.K, x0, sums[2]), rad0, &lo[0], &hi[0]);
+        ry = dim_to_cells_fast(c, 1, y0 + sums[1], filter_error_bound(c.K, y0, sums[3]), rad1, &lo[1], &hi[1]);
     }
-    for (; k < c.K; ++k) {
-        px = px + inc[2 * k];
-        py = py + inc[2 * k + 1];
+    if (rx == kCellsReject || ry == kCellsReject) return false;
+    if (rx == kCellsUndecided || ry == kCellsUndecided) {
+        // Exact path: the reference's step-by-step sum. Scalar loads, eight steps
+        // per wait; the additions stay strictly in step order.
+        double px = x0, py = y0;
+        uint32_t k = 0;
+        for (; k + 8 <= c.K; k += 8) {
+            double tx[8], ty[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                tx[j] = inc[2 * (k + j)];
+                ty[j] = inc[2 * (k + j) + 1];
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                px = px + tx[j];
+                py = py + ty[j];
+            }
+        }
+        for (; k < c.K; ++k) {
+            px = px + inc[2 * k];
+            py = py + inc[2 * k + 1];
+        }
+        if (!dim_to_cells(c, 0, px - rad0, px + rad0, &lo[0], &hi[0]) ||
+            !dim_to_cells(c, 1, py - rad1, py + rad1, &lo[1], &hi[1]))
+            return false;
     }
-    if (!dim_to_cells(c, 0, px - rad0, px + rad0, &lo[0], &hi[0]) ||
-        !dim_to_cells(c, 1, py - rad1, py + rad1, &lo[1], &hi[1]))
-        return false;
     if (c.constraint_type != GSC_CONSTRAINT_NONE) {
         // Every loop of abstraction.hpp:124-156 must be non-empty for a cell to be
         // tested; the test reads x and y only, so it reduces to "no failing (x, y)
@@ -466,7 +500,8 @@ __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
             uint32_t lo[4], hi[4];
             const ConstPtr<double> inc =
                 to_const(reinterpret_cast<const double*>(A.inc + (static_cast<size_t>(plane) * c.I + i) * c.K));
-            const bool ok = state_ok && pair_box(c, inc, b2, b3, x0, y0, rad0, rad1, A.fail2d, lo, hi);
+            const ConstPtr<double> sums = to_const(A.inc_sums + (static_cast<size_t>(plane) * c.I + i) * 4);
+            const bool ok = state_ok && pair_box(c, inc, sums, b2, b3, x0, y0, rad0, rad1, A.fail2d, lo, hi);
             if (ok) {
                 // Inverted ranges on a non-wrap axis or wrapped x/y ranges take the escape path.
                 const bool regular = lo[0] <= hi[0] && lo[1] <= hi[1] && (lo[2] <= hi[2] || wd == 2) &&
@@ -865,8 +900,9 @@ __global__ __launch_bounds__(256) void satisfy_recompute_kernel(RecomputeArgs R)
                 const uint2 b3 = A.box3[iv * c.n_a + ia];
                 const ConstPtr<double> inc =
                     to_const(reinterpret_cast<const double*>(R.inc + (static_cast<size_t>(plane) * c.I + i) * c.K));
+                const ConstPtr<double> sums = to_const(R.inc_sums + (static_cast<size_t>(plane) * c.I + i) * 4);
                 uint32_t lo[4], hi[4];
-                if (!state_ok || !pair_box(c, inc, b2, b3, x0, y0, rad0, rad1, R.fail2d, lo, hi)) continue;
+                if (!state_ok || !pair_box(c, inc, sums, b2, b3, x0, y0, rad0, rad1, R.fail2d, lo, hi)) continue;
                 const bool regular = lo[0] <= hi[0] && lo[1] <= hi[1] && (lo[2] <= hi[2] || wd == 2) &&
                                      (lo[3] <= hi[3] || wd == 3);
                 bool sat;

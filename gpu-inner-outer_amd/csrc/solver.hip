@@ -148,6 +148,7 @@ void gsc_solver::setup() {
         dc.ub[d] = cfg.state_ub[d];
         dc.eta[d] = cfg.state_eta[d];
         dc.lb_minus_eps[d] = cfg.state_lb[d] - kEps;
+        dc.inv_eta[d] = 1.0 / cfg.state_eta[d];
         dc.ub_plus_eps[d] = cfg.state_ub[d] + kEps;
         dc.shape[d] = shape.state_shape[d];
         dc.stride[d] = static_cast<uint32_t>(shape.state_strides[d]);
@@ -253,6 +254,7 @@ void gsc_solver::setup() {
     const uint64_t n_inc = static_cast<uint64_t>(dc.shape[2]) * dc.shape[3] * dc.I * dc.K;
     if (n_inc * sizeof(double2) > (4ull << 30)) throw ConfigError("trajectory increment table exceeds 4 GiB");
     inc = alloc<double2>(n_inc);
+    inc_sums = alloc<double>(4ull * dc.shape[2] * dc.shape[3] * dc.I);
     d_obstacles = alloc<gsc_rect4>(obstacles.size());
     d_targets = alloc<gsc_rect4>(targets.size());
 
@@ -285,6 +287,13 @@ void gsc_solver::setup() {
         hipLaunchKernelGGL(increments_kernel, dim3(blocks_for(n_inc, 256)), dim3(256), 0, stream, dc, cs, vt, inc);
         GSC_HIP(hipGetLastError());
     }
+    {
+        // Also with no substeps (K = 0): the sums are then zero.
+        const uint64_t n_rows = static_cast<uint64_t>(dc.shape[2]) * dc.shape[3] * dc.I;
+        hipLaunchKernelGGL(increment_sums_kernel, dim3(blocks_for(n_rows, 256)), dim3(256), 0, stream, dc, inc,
+                           inc_sums);
+        GSC_HIP(hipGetLastError());
+    }
     // Masks on the device (the reference builds them in a serial host loop even in
     // GPU mode, abstraction.hpp:257-265).
     hipLaunchKernelGGL(masks_kernel, dim3(blocks_for(S, 256)), dim3(256), 0, stream, dc, d_obstacles, d_targets, valid,
@@ -315,7 +324,7 @@ uint64_t gsc_solver::device_bytes(const gsc_case_shape& sh, uint32_t substeps) {
     b += r(4 * planes * I_pad * n0 * n1) + r(4 * planes * I_pad) + r(8 * ((S + 63) / 64 + 1));
     b += r(sizeof(OverflowEntry) * (1u << 16)) + r(4) + r(16) + r(sizeof(IterCtl)) + r(sizeof(IterRecord) * kRing);
     b += r(16 * n2 * nw * K) + r(8 * n3 * na * K) + r(8 * n2 * nw) + r(8 * n3 * na) + r(16 * n3);
-    b += r(4 * (n0 + 1) * (n1 + 1)) + r(16 * planes * I * K) + 2 * r(64 * 64);
+    b += r(4 * (n0 + 1) * (n1 + 1)) + r(16 * planes * I * K) + r(32 * planes * I) + 2 * r(64 * 64);
     return b;
 }
 
@@ -360,6 +369,7 @@ void gsc_solver::launch_abstraction() {
     AbstractionArgs A{};
     A.c = dc;
     A.inc = inc;
+    A.inc_sums = inc_sums;
     A.box2 = box2;
     A.box3 = box3;
     A.rad = rad;
@@ -449,6 +459,7 @@ void gsc_solver::solve() {
     RecomputeArgs RA{};
     RA.sat = A;
     RA.inc = inc;
+    RA.inc_sums = inc_sums;
     RA.rad = rad;
     RA.fail2d = fail2d;
     const uint32_t max_it = cfg.max_iterations;
