@@ -107,7 +107,13 @@ struct OverflowEntry {
 struct DevCase {
     double lb[4], ub[4], eta[4];
     double lb_minus_eps[4];  // lb[d] - kEps   (grid.hpp:193)
-    double inv_eta[4];       // 1 / eta[d], for the filtered cell rounding only
+    // Filtered rounding of the nominal x / y (dims 0, 1; set_cell_filter,
+    // dim_to_cells_fast), in quotient units (coordinate - lb) / eta: the domain
+    // limits lb - kEps, ub + kEps moved out and in by the error bound, the
+    // bound itself and 1 - bound, and 1 / eta.
+    double flt_rej_lo[2], flt_in_lo[2], flt_in_hi[2], flt_rej_hi[2];
+    double flt_q[2], flt_q1[2];
+    double inv_eta[2];
     double ub_plus_eps[4];   // ub[d] + kEps
     double wrap_span_minus_eps;  // (ub[w] - lb[w]) - kEps (grid.hpp:198-207)
     uint32_t shape[4];
@@ -217,52 +223,64 @@ GSC_HD inline bool dim_to_cells(const DevCase& c, int d, double mn, double mx, u
 
 // ---- filtered rounding of the nominal x / y coordinate ----------------------
 // The reference adds the K trajectory increments to the cell centre one at a
-// time, rounding after every step (x <- x + inc_k, model.hpp:40-41). The sum S
-// of the increments is the same for every cell of a (θ, v) plane and input, so
-// p = x0 + S approximates the reference's coordinate: with A = sum |inc_k| and
-// u = 2^-53, K roundings of partial sums bounded by |x0| + A on each side plus
-// the final addition keep them within 2u(K + 1)(|x0| + A) (the caller's `err`).
-// dim_to_cells_fast carries that bound through the box edges (p -/+ rad), the
-// domain test against lb - kEps / ub + kEps and (edge - lb) / eta, widens it by
-// kFilterWiden, and returns the reference's answer only when every threshold —
-// each floor, each ceil, each domain comparison — is clear of it; otherwise
-// kCellsUndecided, and the caller runs the exact step-by-step sum. Non-wrap
-// dimensions only.
+// time, rounding after every step (x <- x + inc_k, model.hpp:40-41), takes the
+// box edges p -/+ rad and rounds (edge - lb) / eta to cells (grid.hpp:181-251).
+// The sum S of the increments and the radius are the same for every cell of a
+// (θ, v) plane and input, so the kernels form the edge quotients as
+//   v0 + (S -/+ rad) / eta,   v0 = (x0 - lb) / eta per cell,
+// with the second term from a per-(plane, input) table. With u = 2^-53 and X
+// bounding |x0| + sum|inc_k| + rad, the reference's sequential sum and S differ
+// from the exact sum by at most uK(|x0| + A) and uKA, and every other rounding
+// on either side (the reference's edge, subtraction and division; here the
+// reciprocal, two products and the sum) is a few u of |lb| + X over eta or of
+// the quotient, so
+//   |q_ref - q| <= ((2K + 10) X + 6|lb|) u / eta + 4u (qmax + 1)
+// where qmax bounds the quotients inside the domain. set_cell_filter widens
+// that by kFilterWiden; dim_to_cells_fast returns the reference's answer only
+// when every threshold — the domain limits lb - kEps / ub + kEps in quotient
+// units, the floor of the lower quotient, the ceil of the upper one — is clear
+// of the bound, and kCellsUndecided otherwise, for the caller to run the exact
+// step-by-step sum. Non-wrap dimensions only.
 constexpr double kUnitRoundoff = 1.1102230246251565e-16;  // 2^-53
 constexpr double kFilterWiden = 1024.0;
 enum : int { kCellsUndecided = 0, kCellsReject = 1, kCellsOk = 2 };
 
-GSC_HD inline double filter_error_bound(uint32_t K, double x0, double abs_sum) {
-    return 2.0 * kUnitRoundoff * (static_cast<double>(K) + 1.0) * (fabs(x0) + abs_sum);
+// Constants of dimension d (0 or 1) for cell centres |x0| <= x_abs, K-step
+// increment sums with sum |inc_k| <= abs_sum, and radii |rad| <= rad_max.
+// Needs lb, eta, shape, lb_minus_eps and ub_plus_eps of dimension d.
+inline void set_cell_filter(DevCase& c, int d, uint32_t K, double x_abs, double abs_sum, double rad_max) {
+    const double u = kUnitRoundoff;
+    const double X = x_abs + abs_sum + rad_max;
+    const double ie = 1.0 / c.eta[d];
+    const double aie = fabs(ie);
+    const double qmax = static_cast<double>(c.shape[d]) + 3.0 + 2.0 * kEps * aie;
+    const double eq = kFilterWiden * (((2.0 * K + 10.0) * X + 6.0 * fabs(c.lb[d])) * u * aie + 4.0 * u * (qmax + 1.0));
+    const double t_lo = (c.lb_minus_eps[d] - c.lb[d]) * ie;
+    const double t_hi = (c.ub_plus_eps[d] - c.lb[d]) * ie;
+    c.inv_eta[d] = ie;
+    c.flt_q[d] = eq;
+    c.flt_q1[d] = 1.0 - eq;
+    c.flt_rej_lo[d] = t_lo - eq;
+    c.flt_in_lo[d] = t_lo + eq;
+    c.flt_in_hi[d] = t_hi - eq;
+    c.flt_rej_hi[d] = t_hi + eq;
 }
 
-GSC_HD inline int dim_to_cells_fast(const DevCase& c, int d, double p, double err, double rad, uint32_t* lo,
+// v0 = (x0 - lb) * inv_eta of the cell; qa, qb = (S - rad) * inv_eta and
+// (S + rad) * inv_eta of the (plane, input).
+GSC_HD inline int dim_to_cells_fast(const DevCase& c, int d, double v0, double qa, double qb, uint32_t* lo,
                                     uint32_t* hi) {
-    const double mn = p - rad, mx = p + rad;
-    // Edges: the coordinate's error plus one rounding on each side.
-    const double e_raw = err + 4.0 * kUnitRoundoff * (fabs(p) + fabs(rad));
-    const double e = kFilterWiden * e_raw;
-    if (mn < c.lb_minus_eps[d] - e || mx > c.ub_plus_eps[d] + e) return kCellsReject;
-    if (mn <= c.lb_minus_eps[d] + e || mx >= c.ub_plus_eps[d] - e) return kCellsUndecided;
-    // Quotients: the edge error over eta, plus the roundings of the subtraction,
-    // of the reference's division and of the reciprocal and product here.
-    const double ql = (mn - c.lb[d]) * c.inv_eta[d];
-    const double qh = (mx - c.lb[d]) * c.inv_eta[d];
-    const double eq = kFilterWiden * ((e_raw + 4.0 * kUnitRoundoff * (fabs(mn) + fabs(mx) + fabs(c.lb[d]))) *
-                                          fabs(c.inv_eta[d]) +
-                                      8.0 * kUnitRoundoff * (1.0 + fabs(ql) + fabs(qh)));
+    const double ql = v0 + qa, qh = v0 + qb;
+    const bool reject = (ql < c.flt_rej_lo[d]) | (qh > c.flt_rej_hi[d]);
+    const bool inside = (ql > c.flt_in_lo[d]) & (qh < c.flt_in_hi[d]);
     const double fl = floor(ql), ce = ceil(qh);
-    if (!(ql - eq > fl && ql + eq < fl + 1.0 && qh - eq > ce - 1.0 && qh + eq < ce)) return kCellsUndecided;
-    int64_t lower = static_cast<int64_t>(fl);
-    int64_t upper = static_cast<int64_t>(ce) - 1;
-    const int64_t n = static_cast<int64_t>(c.shape[d]);
-    if (lower < 0) lower = 0;
-    if (upper < 0) upper = 0;
-    if (lower >= n) lower = n - 1;
-    if (upper >= n) upper = n - 1;
-    *lo = static_cast<uint32_t>(lower);
-    *hi = static_cast<uint32_t>(upper);
-    return kCellsOk;
+    const double f = ql - fl, g = ce - qh;  // distances to the thresholds, in [0, 1)
+    const bool clear = (f > c.flt_q[d]) & (f < c.flt_q1[d]) & (g > c.flt_q[d]) & (g < c.flt_q1[d]);
+    // dim_to_cells' clamps, on the (integral) doubles.
+    const double top = static_cast<double>(c.shape[d]) - 1.0;
+    *lo = static_cast<uint32_t>(fmin(fmax(fl, 0.0), top));
+    *hi = static_cast<uint32_t>(fmin(fmax(ce - 1.0, 0.0), top));
+    return reject ? kCellsReject : (inside & clear) ? kCellsOk : kCellsUndecided;
 }
 
 // 16-corner inclusion–exclusion on a padded u32 prefix table (prefix_sum.hpp:65-85)

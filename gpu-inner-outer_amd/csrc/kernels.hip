@@ -340,26 +340,28 @@ __global__ __launch_bounds__(256) void increments_kernel(DevCase c, const double
     inc[t] = double2{c.dt * dx, c.dt * dy};
 }
 
-// Per (plane, input): the sum of the K increments in step order and the sum of
-// their magnitudes, for x and y — the filtered fast path of pair_box
-// (dim_to_cells_fast). Thread per (plane, input).
+// Per (plane, input), for the filtered fast path of pair_box
+// (dim_to_cells_fast): with S the sum of the K increments in step order and
+// rad the plane's radius, (S - rad) / eta and (S + rad) / eta for x and for y.
+// Thread per (plane, input).
 __global__ __launch_bounds__(256) void increment_sums_kernel(DevCase c, const double2* __restrict__ inc,
-                                                              double* __restrict__ sums) {
+                                                              const double* __restrict__ rad, double* __restrict__ sums) {
     const uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= static_cast<uint64_t>(c.shape[2]) * c.shape[3] * c.I) return;
+    const uint32_t plane = static_cast<uint32_t>(t / c.I);
+    const uint32_t iv = plane / c.shape[2];
     const double2* row = inc + t * c.K;
-    double sx = 0.0, sy = 0.0, ax = 0.0, ay = 0.0;
+    double sx = 0.0, sy = 0.0;
     for (uint32_t k = 0; k < c.K; ++k) {
         const double2 v = row[k];
         sx = sx + v.x;
         sy = sy + v.y;
-        ax = ax + fabs(v.x);
-        ay = ay + fabs(v.y);
     }
-    sums[4 * t] = sx;
-    sums[4 * t + 1] = sy;
-    sums[4 * t + 2] = ax;
-    sums[4 * t + 3] = ay;
+    const double r0 = rad[2 * iv], r1 = rad[2 * iv + 1];
+    sums[4 * t] = (sx - r0) * c.inv_eta[0];
+    sums[4 * t + 1] = (sx + r0) * c.inv_eta[0];
+    sums[4 * t + 2] = (sy - r1) * c.inv_eta[1];
+    sums[4 * t + 3] = (sy + r1) * c.inv_eta[1];
 }
 
 // θ/v part of the probe cell per (plane, input): lo2 * stride2 + lo3 * stride3 of
@@ -404,8 +406,8 @@ __device__ __forceinline__ bool pair_box(const DevCase& c, ConstPtr<double> inc,
     // plain intervals when the wrap axis is θ or v).
     int rx = kCellsUndecided, ry = kCellsUndecided;
     if (c.wrap_dim >= 2) {
-        rx = dim_to_cells_fast(c, 0, x0 + sums[0], filter_error_bound(c.K, x0, sums[2]), rad0, &lo[0], &hi[0]);
-        ry = dim_to_cells_fast(c, 1, y0 + sums[1], filter_error_bound(c.K, y0, sums[3]), rad1, &lo[1], &hi[1]);
+        rx = dim_to_cells_fast(c, 0, (x0 - c.lb[0]) * c.inv_eta[0], sums[0], sums[1], &lo[0], &hi[0]);
+        ry = dim_to_cells_fast(c, 1, (y0 - c.lb[1]) * c.inv_eta[1], sums[2], sums[3], &lo[1], &hi[1]);
     }
     if (rx == kCellsReject || ry == kCellsReject) return false;
     if (rx == kCellsUndecided || ry == kCellsUndecided) {

@@ -55,7 +55,7 @@ struct IterRecord {
 struct AbstractionArgs {
     DevCase c;
     const double2* inc;          // [plane][input][K] nominal-trajectory increments
-    const double* inc_sums;      // [plane][input][4]: sum and sum of |.| of the x / y increments
+    const double* inc_sums;      // [plane][input][4]: (S -/+ rad) / eta for x, then y (increment_sums_kernel)
     const uint2* box2;           // [n_θ][n_ω] θ cell range (or kBoxReject)
     const uint2* box3;           // [n_v][n_a] v cell range (or kBoxReject)
     const double* rad;           // [n_v][2] x / y box radius
@@ -139,7 +139,7 @@ __global__ void prefix_rows_kernel(DevCase c, const uint8_t* a, const uint8_t* b
                                    const IterCtl* ctl);
 __global__ void prefix_axis_kernel(DevCase c, int axis, uint32_t* P0, uint32_t* P1, const IterCtl* ctl);
 __global__ void increments_kernel(DevCase c, const double2* cs, const double* vt, double2* inc);
-__global__ void increment_sums_kernel(DevCase c, const double2* inc, double* sums);
+__global__ void increment_sums_kernel(DevCase c, const double2* inc, const double* rad, double* sums);
 // Coarse sample of a prefix table (with ctl: of table k & 1 into coarse k & 1).
 __global__ void coarse_kernel(DevCase c, const uint32_t* P0, const uint32_t* P1, uint32_t* C0, uint32_t* C1,
                               const IterCtl* ctl);

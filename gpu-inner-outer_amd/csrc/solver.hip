@@ -148,7 +148,6 @@ void gsc_solver::setup() {
         dc.ub[d] = cfg.state_ub[d];
         dc.eta[d] = cfg.state_eta[d];
         dc.lb_minus_eps[d] = cfg.state_lb[d] - kEps;
-        dc.inv_eta[d] = 1.0 / cfg.state_eta[d];
         dc.ub_plus_eps[d] = cfg.state_ub[d] + kEps;
         dc.shape[d] = shape.state_shape[d];
         dc.stride[d] = static_cast<uint32_t>(shape.state_strides[d]);
@@ -241,6 +240,23 @@ void gsc_solver::setup() {
     std::vector<uint2> h_box2, h_box3;
     std::vector<uint32_t> h_fail;
     build_tables(h_cs, h_vt, h_box2, h_box3, h_rad, h_fail);
+    {
+        // Filter constants of the x / y fast path (set_cell_filter): every cell
+        // centre lies within [lb, lb + n eta + eta], every increment within
+        // |dt| max|v| (times three roundings), every radius within the table's.
+        double vmax = 0.0, rmax[2] = {0.0, 0.0};
+        for (const double v : h_vt) vmax = std::max(vmax, std::fabs(v));
+        for (size_t k = 0; k + 1 < h_rad.size(); k += 2) {
+            rmax[0] = std::max(rmax[0], std::fabs(h_rad[k]));
+            rmax[1] = std::max(rmax[1], std::fabs(h_rad[k + 1]));
+        }
+        const double abs_sum = static_cast<double>(dc.K) * std::fabs(dc.dt) * vmax * (1.0 + 1e-12);
+        for (int d = 0; d < 2; ++d) {
+            const double top = dc.lb[d] + static_cast<double>(dc.shape[d]) * dc.eta[d];
+            const double x_abs = std::max(std::fabs(dc.lb[d]), std::fabs(top)) + std::fabs(dc.eta[d]);
+            set_cell_filter(dc, d, dc.K, x_abs, abs_sum, rmax[d]);
+        }
+    }
     cs = alloc<double2>(h_cs.size());
     vt = alloc<double>(h_vt.size());
     box2 = alloc<uint2>(h_box2.size());
@@ -290,7 +306,7 @@ void gsc_solver::setup() {
     {
         // Also with no substeps (K = 0): the sums are then zero.
         const uint64_t n_rows = static_cast<uint64_t>(dc.shape[2]) * dc.shape[3] * dc.I;
-        hipLaunchKernelGGL(increment_sums_kernel, dim3(blocks_for(n_rows, 256)), dim3(256), 0, stream, dc, inc,
+        hipLaunchKernelGGL(increment_sums_kernel, dim3(blocks_for(n_rows, 256)), dim3(256), 0, stream, dc, inc, rad,
                            inc_sums);
         GSC_HIP(hipGetLastError());
     }

@@ -7,6 +7,7 @@
 //
 // Usage: cell_filter_check <trials> <seed>; prints "checked <n> decided <d>
 // mismatches <m>" and exits 1 on any mismatch.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -37,7 +38,6 @@ int main(int argc, char** argv) {
         c.shape[d] = n;
         c.lb_minus_eps[d] = c.lb[d] - kEps;
         c.ub_plus_eps[d] = c.ub[d] + kEps;
-        c.inv_eta[d] = 1.0 / eta;
         c.wrap_dim = 2;
         const uint32_t K = (const uint32_t[]){1, 8, 32, 64}[rng() % 4];
         const double h = std::exp(pick(std::log(1e-6), std::log(1.0)));
@@ -70,14 +70,15 @@ int main(int argc, char** argv) {
         const double p = exact_coord(x0);
         uint32_t elo = 0, ehi = 0;
         const bool eok = dim_to_cells(c, d, p - rad, p + rad, &elo, &ehi);
-        // Filtered path (sums in the device kernel's order).
-        double S = 0.0, A = 0.0;
-        for (uint32_t k = 0; k < K; ++k) {
-            S = S + inc[k];
-            A = A + std::fabs(inc[k]);
-        }
+        // Filtered path, evaluated as the kernels do (sum in step order, then the
+        // quotient terms), with the case-wide bounds the solver would derive:
+        // |x0|, sum |inc_k| <= K h, |rad|.
+        double S = 0.0;
+        for (uint32_t k = 0; k < K; ++k) S = S + inc[k];
+        set_cell_filter(c, d, K, std::max(std::max(std::fabs(c.lb[d]), std::fabs(c.ub[d])), std::fabs(x0)), K * h, rad);
+        const double ie = c.inv_eta[d];
         uint32_t flo = 0, fhi = 0;
-        const int r = dim_to_cells_fast(c, d, x0 + S, filter_error_bound(K, x0, A), rad, &flo, &fhi);
+        const int r = dim_to_cells_fast(c, d, (x0 - lb) * ie, (S - rad) * ie, (S + rad) * ie, &flo, &fhi);
         if (r == kCellsUndecided) continue;
         ++decided;
         if (r == kCellsReject) ++rejected;
