@@ -384,6 +384,41 @@ __global__ __launch_bounds__(256) void probe_zw_kernel(DevCase c, const uint2* _
     probe_zw[t] = v;
 }
 
+// The kernel's own argument block, addressed as ordinary (global) memory, so
+// that out-of-line helpers can read the case from it without the caller
+// keeping its fields in registers.
+template <typename T>
+__device__ inline const T* kernarg_block() {
+    return reinterpret_cast<const T*>(reinterpret_cast<uintptr_t>(__builtin_amdgcn_kernarg_segment_ptr()));
+}
+
+// Exact x / y cells of one pair: the reference's step-by-step sum
+// (model.hpp:40-41, additions strictly in step order) and dim_to_cells.
+// Out of line on purpose: it runs only where the filtered fast path cannot
+// decide, and keeping its constants out of the callers' input loops leaves the
+// scalar registers to the fast path. `cg` points at the case in memory.
+struct ExactCells {
+    uint32_t lo0, hi0, lo1, hi1;
+    uint32_t ok;
+};
+
+__device__ __noinline__ ExactCells exact_cells(const DevCase* __restrict__ cg, const double2* __restrict__ row,
+                                               double x0, double y0, double rad0, double rad1) {
+    const DevCase& c = *cg;
+    double px = x0, py = y0;
+    for (uint32_t k = 0; k < c.K; ++k) {
+        const double2 v = row[k];
+        px = px + v.x;
+        py = py + v.y;
+    }
+    ExactCells e{};
+    e.ok = (dim_to_cells(c, 0, px - rad0, px + rad0, &e.lo0, &e.hi0) &&
+            dim_to_cells(c, 1, py - rad1, py + rad1, &e.lo1, &e.hi1))
+               ? 1u
+               : 0u;
+    return e;
+}
+
 // Successor cell box of one pair, the reference's arithmetic in the reference's order:
 //   centre           grid.hpp:42 (x0, y0 passed in)
 //   nominal x, y     model.hpp:35-42: x += dt*(v_k*cos θ_k), y += dt*(v_k*sin θ_k),
@@ -393,8 +428,8 @@ __global__ __launch_bounds__(256) void probe_zw_kernel(DevCase c, const uint2* _
 //   constraint scan  abstraction.hpp:110-159 as a 2D prefix query over x,y fails
 // Returns false for an invalid pair (abstraction.hpp:187-205). `inc` points at
 // the pair's (plane, input) row of K increments and must be wave-uniform.
-__device__ __forceinline__ bool pair_box(const DevCase& c, ConstPtr<double> inc, ConstPtr<double> sums, uint2 b2,
-                                         uint2 b3, double x0, double y0, double rad0, double rad1,
+__device__ __forceinline__ bool pair_box(const DevCase& c, const DevCase* cg, const double2* inc, ConstPtr<double> sums,
+                                         uint2 b2, uint2 b3, double x0, double y0, double rad0, double rad1,
                                          const uint32_t* __restrict__ fail2d, uint32_t lo[4], uint32_t hi[4]) {
     if (b2.x == kBoxReject || b3.x == kBoxReject) return false;
     lo[2] = b2.x;
@@ -411,30 +446,12 @@ __device__ __forceinline__ bool pair_box(const DevCase& c, ConstPtr<double> inc,
     }
     if (rx == kCellsReject || ry == kCellsReject) return false;
     if (rx == kCellsUndecided || ry == kCellsUndecided) {
-        // Exact path: the reference's step-by-step sum. Scalar loads, eight steps
-        // per wait; the additions stay strictly in step order.
-        double px = x0, py = y0;
-        uint32_t k = 0;
-        for (; k + 8 <= c.K; k += 8) {
-            double tx[8], ty[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                tx[j] = inc[2 * (k + j)];
-                ty[j] = inc[2 * (k + j) + 1];
-            }
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                px = px + tx[j];
-                py = py + ty[j];
-            }
-        }
-        for (; k < c.K; ++k) {
-            px = px + inc[2 * k];
-            py = py + inc[2 * k + 1];
-        }
-        if (!dim_to_cells(c, 0, px - rad0, px + rad0, &lo[0], &hi[0]) ||
-            !dim_to_cells(c, 1, py - rad1, py + rad1, &lo[1], &hi[1]))
-            return false;
+        const ExactCells e = exact_cells(cg, inc, x0, y0, rad0, rad1);
+        if (!e.ok) return false;
+        lo[0] = e.lo0;
+        hi[0] = e.hi0;
+        lo[1] = e.lo1;
+        hi[1] = e.hi1;
     }
     if (c.constraint_type != GSC_CONSTRAINT_NONE) {
         // Every loop of abstraction.hpp:124-156 must be non-empty for a cell to be
@@ -500,10 +517,11 @@ __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
             const uint2 b3 = A.box3[iv * c.n_a + ia];
             uint32_t r = kRecInvalid;
             uint32_t lo[4], hi[4];
-            const ConstPtr<double> inc =
-                to_const(reinterpret_cast<const double*>(A.inc + (static_cast<size_t>(plane) * c.I + i) * c.K));
+            const double2* const inc = A.inc + (static_cast<size_t>(plane) * c.I + i) * c.K;
             const ConstPtr<double> sums = to_const(A.inc_sums + (static_cast<size_t>(plane) * c.I + i) * 4);
-            const bool ok = state_ok && pair_box(c, inc, sums, b2, b3, x0, y0, rad0, rad1, A.fail2d, lo, hi);
+            const bool ok =
+                state_ok && pair_box(c, &kernarg_block<AbstractionArgs>()->c, inc, sums, b2, b3, x0, y0, rad0, rad1,
+                                     A.fail2d, lo, hi);
             if (ok) {
                 // Inverted ranges on a non-wrap axis or wrapped x/y ranges take the escape path.
                 const bool regular = lo[0] <= hi[0] && lo[1] <= hi[1] && (lo[2] <= hi[2] || wd == 2) &&
@@ -900,11 +918,12 @@ __global__ __launch_bounds__(256) void satisfy_recompute_kernel(RecomputeArgs R)
                 if (!pending) continue;
                 ++tested;
                 const uint2 b3 = A.box3[iv * c.n_a + ia];
-                const ConstPtr<double> inc =
-                    to_const(reinterpret_cast<const double*>(R.inc + (static_cast<size_t>(plane) * c.I + i) * c.K));
+                const double2* const inc = R.inc + (static_cast<size_t>(plane) * c.I + i) * c.K;
                 const ConstPtr<double> sums = to_const(R.inc_sums + (static_cast<size_t>(plane) * c.I + i) * 4);
                 uint32_t lo[4], hi[4];
-                if (!state_ok || !pair_box(c, inc, sums, b2, b3, x0, y0, rad0, rad1, R.fail2d, lo, hi)) continue;
+                if (!state_ok || !pair_box(c, &kernarg_block<RecomputeArgs>()->sat.c, inc, sums, b2, b3, x0, y0, rad0,
+                                           rad1, R.fail2d, lo, hi))
+                    continue;
                 const bool regular = lo[0] <= hi[0] && lo[1] <= hi[1] && (lo[2] <= hi[2] || wd == 2) &&
                                      (lo[3] <= hi[3] || wd == 3);
                 bool sat;
