@@ -272,6 +272,123 @@ __global__ __launch_bounds__(256) void prefix_axis_kernel(DevCase c, int axis, u
     prefix_axis_body(c, axis, P, blockIdx.x);
 }
 
+// ---------------------------------------------------------------------------
+// Fused prefix passes (planes that fit in LDS): three launches per table build
+// instead of rows, three axes, coarse sample and bitmask.
+//
+// prefix_plane_kernel: one 1024-thread workgroup per (θ, v) plane. It scans
+// valid ∧ ¬reached along x (a wave per row, straight from the masks) and then
+// along y (a wave per column) in LDS, and writes the plane's 2D prefix into the
+// padded table; the θ and v passes follow (prefix_axis_kernel with axis 2,
+// prefix_last_axis_kernel). It also writes the plane's words of the unreached
+// bitmask: word k belongs to the plane holding state 64k and is read straight
+// from the masks, including any states of the next plane it covers.
+// LDS layout: element (x, y) at y * pitch + (x ^ (y & 63)), pitch a multiple
+// of 64, so that rows (lanes along x) and columns (lanes along y) both hit 64
+// different banks.
+__global__ __launch_bounds__(kPlaneThreads) void prefix_plane_kernel(DevCase c, const uint8_t* __restrict__ a,
+                                                                      const uint8_t* __restrict__ b, uint32_t* P0,
+                                                                      uint32_t* P1,
+                                                                      unsigned long long* __restrict__ bits,
+                                                                      uint32_t pitch, const IterCtl* ctl) {
+    extern __shared__ uint32_t sm[];
+    uint32_t* P = P0;
+    if (ctl) {
+        const uint64_t st = loop_state(ctl);
+        if (state_done(st)) return;
+        P = (state_next_iteration(st) & 1u) ? P1 : P0;
+    }
+    constexpr uint32_t kW = kPlaneThreads / 64;
+    const uint32_t lane = lane_id();
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t plane = blockIdx.x;
+    const uint32_t n0 = c.shape[0], n1 = c.shape[1], PQ = c.PQ;
+    const uint32_t base = plane * PQ;
+    if (bits) {
+        const uint32_t k_begin = (base + 63u) / 64u;
+        const uint32_t k_end = min((base + PQ + 63u) / 64u, (c.S + 63u) / 64u);
+        for (uint32_t k = k_begin + wave; k < k_end; k += kW) {
+            const uint32_t s = k * 64u + lane;
+            const uint64_t m = __ballot(s < c.S && a[s] != 0 && b[s] == 0);
+            if (lane == 0) bits[k] = m;
+        }
+    }
+    // x: a wave per row, from the masks.
+    for (uint32_t y = wave; y < n1; y += kW) {
+        const uint32_t sw = y & 63u;
+        uint32_t carry = 0;
+        for (uint32_t x0 = 0; x0 < n0; x0 += 64u) {
+            const uint32_t x = x0 + lane;
+            const uint32_t q = base + y * n0 + x;
+            uint32_t v = (x < n0 && a[q] != 0 && !(b && b[q] != 0)) ? 1u : 0u;
+            v = wave_inclusive_scan(v) + carry;
+            if (x < n0) sm[y * pitch + (x ^ sw)] = v;
+            carry = __shfl(v, 63, 64);
+        }
+    }
+    __syncthreads();
+    // y: a wave per column, lanes along y.
+    for (uint32_t x = wave; x < n0; x += kW) {
+        uint32_t carry = 0;
+        for (uint32_t y0 = 0; y0 < n1; y0 += 64u) {
+            const uint32_t y = y0 + lane;
+            const uint32_t at = y * pitch + (x ^ (y & 63u));
+            uint32_t v = y < n1 ? sm[at] : 0u;
+            v = wave_inclusive_scan(v) + carry;
+            if (y < n1) sm[at] = v;
+            carry = __shfl(v, 63, 64);
+        }
+    }
+    __syncthreads();
+    const uint32_t z = plane % c.shape[2], w = plane / c.shape[2];
+    const uint32_t zw = (z + 1) * c.pstride[2] + (w + 1) * c.pstride[3];
+    for (uint32_t y = wave; y < n1; y += kW) {
+        const uint32_t row = zw + (y + 1) * c.pstride[1] + 1;
+        for (uint32_t x = lane; x < n0; x += 64u) P[row + x] = sm[y * pitch + (x ^ (y & 63u))];
+    }
+}
+
+// The last (v) axis pass with the coarse sample fused: the thread of line
+// (x, y, θ) writes C[X, Y, θ + 1, v + 1] for every v when x + 1 and y + 1 are
+// coarse sample positions (multiples of kCoarse, or the last index); the
+// coarse entries on the padding planes stay zero from set-up.
+__global__ __launch_bounds__(256) void prefix_last_axis_kernel(DevCase c, uint32_t* P0, uint32_t* P1, uint32_t* C0,
+                                                                uint32_t* C1, const IterCtl* ctl) {
+    uint32_t* P = P0;
+    uint32_t* C = C0;
+    if (ctl) {
+        const uint64_t st = loop_state(ctl);
+        if (state_done(st)) return;
+        const bool odd = (state_next_iteration(st) & 1u) != 0;
+        P = odd ? P1 : P0;
+        C = odd ? C1 : C0;
+    }
+    const uint32_t tid = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t n0 = c.shape[0], n1 = c.shape[1];
+    const uint32_t lines = n0 * n1 * c.shape[2];
+    if (tid >= lines) return;
+    const uint32_t x = tid % n0;
+    const uint32_t r = tid / n0;
+    const uint32_t y = r % n1;
+    const uint32_t z = r / n1;
+    const uint32_t step = c.pstride[3];
+    uint32_t off = (x + 1) + (y + 1) * c.pstride[1] + (z + 1) * c.pstride[2] + step;
+    const bool sx = (x + 1) % kCoarse == 0 || x + 1 == n0;
+    const bool sy = (y + 1) % kCoarse == 0 || y + 1 == n1;
+    const bool sample = sx && sy;
+    uint32_t coff = 0;
+    if (sample)
+        coff = (x + kCoarse) / kCoarse + (y + kCoarse) / kCoarse * c.cstride[1] + (z + 1) * c.cstride[2] + c.cstride[3];
+    uint32_t acc = 0;
+    for (uint32_t k = 0; k < c.shape[3]; ++k) {
+        acc += P[off];
+        P[off] = acc;
+        if (sample) C[coff] = acc;
+        off += step;
+        coff += c.cstride[3];
+    }
+}
+
 // Coarse sample: C[X, Y, z, w] = P[min(kCoarse X, n_x), min(kCoarse Y, n_y), z, w]
 // over the padded θ / v indices. Thread per coarse entry.
 __device__ __forceinline__ void coarse_body(const DevCase& c, const uint32_t* __restrict__ P, uint32_t* __restrict__ C,

@@ -139,6 +139,13 @@ __global__ void prefix_rows_kernel(DevCase c, const uint8_t* a, const uint8_t* b
                                    const IterCtl* ctl);
 __global__ void prefix_axis_kernel(DevCase c, int axis, uint32_t* P0, uint32_t* P1, const IterCtl* ctl);
 __global__ void increments_kernel(DevCase c, const double2* cs, const double* vt, double2* inc);
+// Fused prefix build for planes that fit in LDS (see kernels.hip).
+constexpr uint32_t kPlaneThreads = 1024;
+constexpr uint32_t kPlaneLdsMax = 65536;  // bytes of LDS for one plane's 2D prefix
+__global__ void prefix_plane_kernel(DevCase c, const uint8_t* a, const uint8_t* b, uint32_t* P0, uint32_t* P1,
+                                    unsigned long long* bits, uint32_t pitch, const IterCtl* ctl);
+__global__ void prefix_last_axis_kernel(DevCase c, uint32_t* P0, uint32_t* P1, uint32_t* C0, uint32_t* C1,
+                                        const IterCtl* ctl);
 __global__ void increment_sums_kernel(DevCase c, const double2* inc, const double* rad, double* sums);
 // Coarse sample of a prefix table (with ctl: of table k & 1 into coarse k & 1).
 __global__ void coarse_kernel(DevCase c, const uint32_t* P0, const uint32_t* P1, uint32_t* C0, uint32_t* C1,

@@ -294,6 +294,13 @@ void gsc_solver::setup() {
     GSC_HIP(hipMemsetAsync(prefix_valid, 0, PT * sizeof(uint32_t), stream));
     GSC_HIP(hipMemsetAsync(prefix_unreached[0], 0, PT * sizeof(uint32_t), stream));
     GSC_HIP(hipMemsetAsync(prefix_unreached[1], 0, PT * sizeof(uint32_t), stream));
+    GSC_HIP(hipMemsetAsync(coarse_unreached[0], 0, coarse_count * sizeof(uint32_t), stream));
+    GSC_HIP(hipMemsetAsync(coarse_unreached[1], 0, coarse_count * sizeof(uint32_t), stream));
+    {
+        // The fused prefix build keeps one plane's 2D prefix in LDS.
+        const uint32_t pitch = (dc.shape[0] + 63u) / 64u * 64u;
+        plane_pitch = static_cast<uint64_t>(pitch) * dc.shape[1] * sizeof(uint32_t) <= kPlaneLdsMax ? pitch : 0u;
+    }
     GSC_HIP(hipMemsetAsync(mask_counts, 0, 2 * sizeof(unsigned long long), stream));
 
     hipLaunchKernelGGL(probe_zw_kernel, dim3(blocks_for(n_planes * dc.I_pad, 256)), dim3(256), 0, stream, dc, box2,
@@ -488,11 +495,28 @@ void gsc_solver::solve() {
         for (uint32_t it = first; it <= last; ++it) {
             hipEvent_t* ev = &events[(it - first) * kEventsPerIter];
             if (timed) GSC_HIP(hipEventRecord(ev[0], stream));
-            build_prefix(valid, reach, prefix_unreached[0], prefix_unreached[1], ctl);
-            build_coarse(prefix_unreached[0], prefix_unreached[1], coarse_unreached[0], coarse_unreached[1], ctl);
-            hipLaunchKernelGGL(unreached_bits_kernel, dim3(blocks_for(S, 256)), dim3(256), 0, stream, S, valid, reach,
-                               ubits, ctl);
-            GSC_HIP(hipGetLastError());
+            if (plane_pitch) {
+                // Plane kernel (x, y and the bitmask), θ pass, v pass with the coarse sample.
+                hipLaunchKernelGGL(prefix_plane_kernel, dim3(dc.shape[2] * dc.shape[3]), dim3(kPlaneThreads),
+                                   plane_pitch * dc.shape[1] * sizeof(uint32_t), stream, dc, valid, reach,
+                                   prefix_unreached[0], prefix_unreached[1], ubits, plane_pitch, ctl);
+                GSC_HIP(hipGetLastError());
+                const uint64_t lines2 = static_cast<uint64_t>(dc.shape[0]) * dc.shape[1] * dc.shape[3];
+                hipLaunchKernelGGL(prefix_axis_kernel, dim3(blocks_for(lines2, 256)), dim3(256), 0, stream, dc, 2,
+                                   prefix_unreached[0], prefix_unreached[1], ctl);
+                GSC_HIP(hipGetLastError());
+                const uint64_t lines3 = static_cast<uint64_t>(dc.shape[0]) * dc.shape[1] * dc.shape[2];
+                hipLaunchKernelGGL(prefix_last_axis_kernel, dim3(blocks_for(lines3, 256)), dim3(256), 0, stream, dc,
+                                   prefix_unreached[0], prefix_unreached[1], coarse_unreached[0], coarse_unreached[1],
+                                   ctl);
+                GSC_HIP(hipGetLastError());
+            } else {
+                build_prefix(valid, reach, prefix_unreached[0], prefix_unreached[1], ctl);
+                build_coarse(prefix_unreached[0], prefix_unreached[1], coarse_unreached[0], coarse_unreached[1], ctl);
+                hipLaunchKernelGGL(unreached_bits_kernel, dim3(blocks_for(S, 256)), dim3(256), 0, stream, S, valid,
+                                   reach, ubits, ctl);
+                GSC_HIP(hipGetLastError());
+            }
             if (timed) GSC_HIP(hipEventRecord(ev[1], stream));
             if (opts.recompute_abstraction) {
                 hipLaunchKernelGGL(satisfy_recompute_kernel, dim3(satisfy_blocks(dc, 256)), dim3(256), 0, stream, RA);
