@@ -958,7 +958,6 @@ __global__ __launch_bounds__(256) void satisfy_recompute_kernel(RecomputeArgs R)
     __shared__ uint32_t s_q[256];
     __shared__ uint8_t s_flags[256];
     __shared__ uint32_t s_wave_n[4];
-    __shared__ uint32_t s_counts[4][5];
     const uint32_t lane = lane_id();
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t PQ = c.PQ;
@@ -1088,30 +1087,16 @@ __global__ __launch_bounds__(256) void satisfy_recompute_kernel(RecomputeArgs R)
         tested += __shfl_xor(tested, off, 64);
         counted += __shfl_xor(counted, off, 64);
     }
-    if (lane == 0) {
-        s_counts[wave][0] = nr;
-        s_counts[wave][1] = nc;
-        s_counts[wave][2] = tested;
-        s_counts[wave][3] = counted;
-        s_counts[wave][4] = slots;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long a = 0, b = 0, t = 0, n = 0, sl = 0;
-        for (uint32_t w = 0; w < 4; ++w) {
-            a += s_counts[w][0];
-            b += s_counts[w][1];
-            t += s_counts[w][2];
-            n += s_counts[w][3];
-            sl += s_counts[w][4];
-        }
-        IterCounters& slot = A.ctl->ring[iter % kRing];
-        const uint32_t k = blockIdx.x % kCounterSlots;
-        if (a) atomicAdd(&slot.new_reachable[k], a);
-        if (b) atomicAdd(&slot.new_certified[k], b);
-        if (t) atomicAdd(&slot.pairs_tested[k], t);
-        if (n) atomicAdd(&slot.boxes_counted[k], n);
-        if (sl) atomicAdd(&slot.test_slots[k], sl);
+    // Each wave adds its own counts: no closing barrier, so waves left without
+    // states after the compaction end at once (as in satisfy_kernel).
+    if (lane == 0 && (nr | nc | tested | counted | slots) != 0) {
+        IterCounters& ring = A.ctl->ring[iter % kRing];
+        const uint32_t k = (blockIdx.x * 4u + wave) % kCounterSlots;
+        if (nr) atomicAdd(&ring.new_reachable[k], static_cast<unsigned long long>(nr));
+        if (nc) atomicAdd(&ring.new_certified[k], static_cast<unsigned long long>(nc));
+        if (tested) atomicAdd(&ring.pairs_tested[k], static_cast<unsigned long long>(tested));
+        if (counted) atomicAdd(&ring.boxes_counted[k], static_cast<unsigned long long>(counted));
+        if (slots) atomicAdd(&ring.test_slots[k], static_cast<unsigned long long>(slots));
     }
 }
 
