@@ -56,6 +56,23 @@ def test_small_random(gpu, oracle, seed):
     _check(c, oracle)
 
 
+@pytest.mark.parametrize("n0,n1", [(100, 128), (100, 129), (200, 70)])
+def test_plane_size_limit(gpu, oracle, n0, n1):
+    # The solver builds the x/y prefix of a (θ, v) plane in LDS when the plane,
+    # with rows padded to 64 cells, fits in 64 KiB (100x128 exactly fits), and
+    # by separate row / column passes otherwise (100x129, 200x70).
+    lb, ub, eta = cases._domain((n0, n1, 24, 3))
+    ilb, iub, ieta = cases._inputs(3)
+    c = cases.Case(
+        name=f"plane_{n0}x{n1}", state_lb=lb, state_ub=ub, state_eta=eta, input_lb=ilb, input_ub=iub,
+        input_eta=ieta, candidate=cases._r((4.0, 4.0, -cases.PI_LITERAL, 0.0), (6.0, 6.0, cases.PI_LITERAL, 2.0)),
+        map=cases._r(lb, ub), obstacles=[cases._r((1.0, 7.0, lb[2], lb[3]), (3.0, 8.0, ub[2], ub[3]))],
+        sample_time=0.5, integration_substeps=32, max_iterations=25, disturbance_half_width=(0.0, 0.0, 0.0, 0.0),
+    )
+    got, _ = _check(c, oracle)
+    assert got.iterations == 25
+
+
 def test_sweep_all_pairs_schedule(gpu, oracle):
     # The reference's schedule (every pair tested every iteration) must give the same result.
     c = cases.unicycle_30x30x24x10(sample_time=0.5)
