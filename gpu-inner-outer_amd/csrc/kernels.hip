@@ -17,6 +17,8 @@
 // streams one 4-byte record per pair (coalesced) and gathers prefix corners;
 // the abstraction pass is scalar FP64 with the trigonometry read from
 // host-built tables.
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace gsc_amd {
@@ -934,13 +936,34 @@ __device__ __forceinline__ bool chunk_changed(const SatisfyArgs& A, uint4 ch, ui
 // Seven waves per SIMD (72 VGPRs, no spill) measured faster than eight with
 // spills and than the unbounded 82 VGPRs: the pass waits on memory, so
 // occupancy pays until spill traffic eats it.
+// The host passes the iteration it enqueues, which is the loop state's next
+// iteration unless the loop has stopped: the body loads the state and the
+// group's chunk flags together and returns on a stopped loop before touching
+// any state.
 template <uint32_t WG, uint32_t IB>
-__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) void satisfy_kernel(SatisfyArgs A) {
-    const uint64_t st = loop_state(A.ctl);
-    if (state_done(st)) return;
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) void satisfy_kernel(SatisfyArgs A,
+                                                                                              uint32_t iter) {
     const uint32_t block = blockIdx.x;
-    const uint32_t iter = state_next_iteration(st);
+    constexpr bool kChunkFlags = true;
 #include "satisfy_body.inc"
+}
+
+// Chunk flags of iteration `iter`: one lane per 64-cell chunk, 1 where the
+// chunk's hull box (x, y from its cells' hulls, θ and v from its plane) holds a
+// state whose unreached count changed since the previous iteration. The
+// satisfaction workgroups read their chunks' flags first and end at once when
+// none is set, without reading their states. Padding bytes keep their zeros.
+__global__ __launch_bounds__(256) void chunk_flags_kernel(SatisfyArgs A, uint8_t* __restrict__ flags,
+                                                          uint32_t iter) {
+    if (loop_done(A.ctl)) return;
+    const DevCase& c = A.c;
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k >= c.n_tasks) return;
+    const uint32_t plane = k / c.chunks_per_plane;
+    const uint32_t j = k - plane * c.chunks_per_plane;
+    const uint32_t* const CU = A.coarse_unreached[iter & 1u];
+    const uint32_t* const Cprev = iter == 1 ? A.coarse_valid : A.coarse_unreached[(iter - 1u) & 1u];
+    flags[plane * chunk_flag_pitch(c) + j] = chunk_changed(A, A.chunk_hull[k], plane, CU, Cprev) ? 1u : 0u;
 }
 
 // Pair satisfaction + first-input reduction with the successor boxes recomputed
@@ -1166,6 +1189,7 @@ __global__ __launch_bounds__(256) void coarse_batch_kernel(const BatchInstance* 
 // function boundary gives the better register allocation (74 vs 97 VGPRs).
 template <uint32_t WG, uint32_t IB>
 __device__ __forceinline__ void satisfy_in_batch(const SatisfyArgs& A, uint32_t block, uint32_t iter) {
+    constexpr bool kChunkFlags = false;  // chunk hulls tested in the body
 #include "satisfy_body.inc"
 }
 
@@ -1216,7 +1240,7 @@ __global__ __launch_bounds__(256) void seed_batch_kernel(const BatchInstance* __
 // ---------------------------------------------------------------------------
 // Host launchers for the tile variants.
 namespace {
-using SatSingle = void (*)(SatisfyArgs);
+using SatSingle = void (*)(SatisfyArgs, uint32_t);
 using SatBatch = void (*)(const BatchInstance*, const uint32_t*, uint32_t);
 
 template <uint32_t WG>
@@ -1237,9 +1261,14 @@ uint32_t satisfy_blocks(const DevCase& c, uint32_t wg) {
     return (c.PQ + wg - 1) / wg * c.shape[2] * c.shape[3];
 }
 
-hipError_t launch_satisfy(const SatisfyArgs& A, uint32_t wg, uint32_t ib, hipStream_t stream) {
+hipError_t launch_satisfy(const SatisfyArgs& A, uint32_t iter, uint32_t wg, uint32_t ib, hipStream_t stream) {
     const SatSingle f = wg == 128 ? pick_single<128>(ib) : wg == 512 ? pick_single<512>(ib) : pick_single<256>(ib);
-    hipLaunchKernelGGL(f, dim3(satisfy_blocks(A.c, wg)), dim3(wg), 0, stream, A);
+    hipLaunchKernelGGL(f, dim3(satisfy_blocks(A.c, wg)), dim3(wg), 0, stream, A, iter);
+    return hipGetLastError();
+}
+
+hipError_t launch_chunk_flags(const SatisfyArgs& A, uint8_t* flags, uint32_t iter, hipStream_t stream) {
+    hipLaunchKernelGGL(chunk_flags_kernel, dim3((A.c.n_tasks + 255u) / 256u), dim3(256), 0, stream, A, flags, iter);
     return hipGetLastError();
 }
 

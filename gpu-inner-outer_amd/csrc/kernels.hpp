@@ -80,6 +80,9 @@ struct SatisfyArgs {
     const uint32_t* coarse_valid;         // coarse sample of prefix_valid
     const uint32_t* hull;   // [S] state hulls
     const uint4* chunk_hull;  // [planes * chunks_per_plane] absolute x0, x1, y0, y1 of 64 cells
+    // [planes * chunk_flag_pitch] 1 where the chunk's hull saw a change since the
+    // previous iteration (chunk_flags_kernel); the single-case solver only.
+    const uint8_t* chunk_flags;
     const uint2* hull2;     // [n_θ] union of the θ ranges over ω (lo > hi: wraps)
     const uint2* hull3;     // [n_v] union of the v ranges over a
     const unsigned long long* unreached_bits;
@@ -164,7 +167,12 @@ __global__ void finish_kernel(IterCtl* ctl, IterRecord* records, uint32_t max_it
 // block of record loads.
 bool satisfy_variant_supported(uint32_t wg, uint32_t ib);
 uint32_t satisfy_blocks(const DevCase& c, uint32_t wg);
-hipError_t launch_satisfy(const SatisfyArgs& A, uint32_t wg, uint32_t ib, hipStream_t stream);
+hipError_t launch_satisfy(const SatisfyArgs& A, uint32_t iter, uint32_t wg, uint32_t ib, hipStream_t stream);
+// Chunk flags: row pitch per plane (a multiple of 16, so that a workgroup of up
+// to 16 waves reads its flags as one aligned word) and the launch that fills
+// them for iteration `iter` from the chunk hulls and the coarse tables.
+__host__ __device__ inline uint32_t chunk_flag_pitch(const DevCase& c) { return (c.chunks_per_plane + 15u) & ~15u; }
+hipError_t launch_chunk_flags(const SatisfyArgs& A, uint8_t* flags, uint32_t iter, hipStream_t stream);
 __global__ void satisfy_recompute_kernel(RecomputeArgs R);
 
 // Multi-instance stages: `starts` holds n + 1 block offsets per stage.

@@ -217,6 +217,7 @@ void gsc_solver::setup() {
     hull = alloc<uint32_t>(S);
     chunk_hull = alloc<uint4>(dc.n_tasks);
     const uint64_t n_planes = static_cast<uint64_t>(shape.state_shape[2]) * shape.state_shape[3];
+    chunk_flags = alloc<uint8_t>(n_planes * chunk_flag_pitch(dc));
     // Recompute mode keeps no per-pair state at all.
     if (!opts.recompute_abstraction) rec = alloc<uint32_t>(n_planes * dc.I_pad * dc.PQ);
     probe_zw = alloc<uint32_t>(n_planes * dc.I_pad);
@@ -296,6 +297,8 @@ void gsc_solver::setup() {
     GSC_HIP(hipMemsetAsync(prefix_unreached[1], 0, PT * sizeof(uint32_t), stream));
     GSC_HIP(hipMemsetAsync(coarse_unreached[0], 0, coarse_count * sizeof(uint32_t), stream));
     GSC_HIP(hipMemsetAsync(coarse_unreached[1], 0, coarse_count * sizeof(uint32_t), stream));
+    // The padding bytes of each plane's chunk-flag row stay zero.
+    GSC_HIP(hipMemsetAsync(chunk_flags, 0, static_cast<uint64_t>(n_planes) * chunk_flag_pitch(dc), stream));
     {
         // The fused prefix build keeps one plane's 2D prefix in LDS.
         const uint32_t pitch = (dc.shape[0] + 63u) / 64u * 64u;
@@ -343,6 +346,7 @@ uint64_t gsc_solver::device_bytes(const gsc_case_shape& sh, uint32_t substeps) {
     const uint64_t planes = n2 * n3, K = substeps;
     uint64_t b = 3 * r(S) + 3 * r(4 * S) + 3 * r(4 * PT) + r(8 * n2) + r(8 * n3);
     b += r(16 * planes * ((n0 * n1 + 63) / 64));
+    b += r(planes * (((n0 * n1 + 63) / 64 + 15) / 16 * 16));
     b += 3 * r(4 * ((n0 + kCoarse - 1) / kCoarse + 1) * ((n1 + kCoarse - 1) / kCoarse + 1) * (n2 + 1) * (n3 + 1));
     b += r(4 * planes * I_pad * n0 * n1) + r(4 * planes * I_pad) + r(8 * ((S + 63) / 64 + 1));
     b += r(sizeof(OverflowEntry) * (1u << 16)) + r(4) + r(16) + r(sizeof(IterCtl)) + r(sizeof(IterRecord) * kRing);
@@ -463,6 +467,7 @@ void gsc_solver::solve() {
     A.coarse_valid = coarse_valid;
     A.hull = hull;
     A.chunk_hull = chunk_hull;
+    A.chunk_flags = chunk_flags;
     A.hull2 = hull2;
     A.hull3 = hull3;
     A.prefix_valid = prefix_valid;
@@ -485,6 +490,7 @@ void gsc_solver::solve() {
     RA.inc_sums = inc_sums;
     RA.rad = rad;
     RA.fail2d = fail2d;
+    const bool chunk_flags_used = !opts.recompute_abstraction && !opts.sweep_all_pairs && dc.hull_enabled;
     const uint32_t max_it = cfg.max_iterations;
     uint32_t done = 0;
     uint32_t finished = 0;
@@ -517,12 +523,15 @@ void gsc_solver::solve() {
                                    reach, ubits, ctl);
                 GSC_HIP(hipGetLastError());
             }
+            // Which 64-cell chunks saw a change in their hull, for the stored-record
+            // satisfaction's frontier skipping.
+            if (chunk_flags_used) GSC_HIP(launch_chunk_flags(A, chunk_flags, it, stream));
             if (timed) GSC_HIP(hipEventRecord(ev[1], stream));
             if (opts.recompute_abstraction) {
                 hipLaunchKernelGGL(satisfy_recompute_kernel, dim3(satisfy_blocks(dc, 256)), dim3(256), 0, stream, RA);
                 GSC_HIP(hipGetLastError());
             } else {
-                GSC_HIP(launch_satisfy(A, tile_wg, tile_ib, stream));
+                GSC_HIP(launch_satisfy(A, it, tile_wg, tile_ib, stream));
             }
             if (timed) GSC_HIP(hipEventRecord(ev[2], stream));
             hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(64), 0, stream, ctl, records, max_it);

@@ -225,6 +225,7 @@ struct gsc_solver {
     uint64_t coarse_count = 0;                            // entries per coarse table
     uint32_t* hull = nullptr;                             // [S] state hulls
     uint4* chunk_hull = nullptr;                          // [n_tasks] 64-cell chunk hulls
+    uint8_t* chunk_flags = nullptr;                       // [planes * chunk_flag_pitch] per iteration
     uint2 *hull2 = nullptr, *hull3 = nullptr;             // θ / v hulls per cell
     uint32_t* rec = nullptr;
     OverflowEntry* overflow = nullptr;
