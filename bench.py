@@ -171,8 +171,9 @@ def measure_traffic(workload_name, kernel, sweep_all, outdir):
         if n == 0 or child is None:
             raise RuntimeError(f"{kernel} not seen in the {counter} pass")
         # The satisfaction kernel is launched in batches of 8; launches after the
-        # fixpoint has stopped return at once. Average over the iterations run.
-        units = child["config"]["iterations"] if kernel == "satisfy_kernel" else n
+        # fixpoint has stopped return at once. Average over the iterations run
+        # (every solve of the child, untimed and timed).
+        units = child["iterations_run"] if kernel == "satisfy_kernel" else n
         out[counter] = {"kib_total": kib, "dispatches": n, "launches_counted": units,
                         "copy_kernel_dispatches": n_copy, "copy_kib_per_gib": kib_copy / COPY_REPEATS,
                         "factor": factor, "bytes_per_launch": kib * 1024.0 * factor / units}
@@ -311,31 +312,51 @@ def main():
     case = workload(args.workload)
     print(f"[bench] workload {case.name}: {case.state_count()} states x {case.input_count()} inputs", file=sys.stderr,
           flush=True)
-    solver = Solver(case, time_kernels=True, sweep_all_pairs=args.sweep_all_pairs,
+    K = args.steps
+    iterations_run = 0  # satisfaction launches that did work, over both solvers (the traffic passes divide by it)
+    # End to end: abstraction + fixpoint as a caller runs them, with no HIP
+    # events between the kernels (each event record costs the stream a marker
+    # packet and a gap of several microseconds).
+    solver = Solver(case, time_kernels=False, sweep_all_pairs=args.sweep_all_pairs,
                     recompute_abstraction=args.recompute_abstraction)
     P = solver.pair_count
     for w in range(args.warmup):
         solver.build_abstraction()
         s = solver.solve()
+        iterations_run += s["iterations"]
         print(f"[bench] warmup {w}: {s['iterations']} iterations, {s['message']}", file=sys.stderr, flush=True)
-
-    abs_ms, solve_ms, sat_ms, prefix_ms, red_ms, iters, tested, counted, slots = 0.0, 0.0, 0.0, 0.0, 0.0, 0, 0, 0, 0
+    abs_ms, solve_ms, iters = 0.0, 0.0, 0
     t0 = time.perf_counter()
-    for k in range(args.steps):
-        a = solver.build_abstraction()
+    for k in range(K):
+        abs_ms += solver.build_abstraction()
         s = solver.solve()
-        abs_ms += a
         solve_ms += s["solve_ms"]
+        iters += s["iterations"]
+    wall_ms = (time.perf_counter() - t0) * 1000.0
+    iterations_run += iters
+    summary = s
+    solver.close()
+    # Per-kernel breakdown: the same solves with HIP events around every
+    # kernel of every iteration.
+    solver = Solver(case, time_kernels=True, sweep_all_pairs=args.sweep_all_pairs,
+                    recompute_abstraction=args.recompute_abstraction)
+    solver.build_abstraction()
+    iterations_run += solver.solve()["iterations"]
+    sat_ms, prefix_ms, red_ms, timed_solve_ms, t_iters, tested, counted, slots = 0.0, 0.0, 0.0, 0.0, 0, 0, 0, 0
+    for k in range(K):
+        solver.build_abstraction()
+        s = solver.solve()
+        timed_solve_ms += s["solve_ms"]
         sat_ms += s["satisfaction_ms"]
         prefix_ms += s["prefix_ms"]
         red_ms += s["reduction_ms"]
-        iters += s["iterations"]
+        t_iters += s["iterations"]
         tested += s["pairs_tested"]
         counted += s["boxes_counted"]
         slots += s["test_slots"]
-    wall_ms = (time.perf_counter() - t0) * 1000.0
-    summary = s
-    K = args.steps
+    iterations_run += t_iters
+    if t_iters != iters:
+        raise RuntimeError(f"timed solves ran {t_iters} iterations, untimed {iters}")
     gpairs = P * iters / (sat_ms / 1000.0) / 1e9
     # Dominant kernel per step decides the roofline line. Algorithmic bytes are
     # the reference pair record (u32 min_flat + u32 max_flat + u8 valid = 9 B)
@@ -387,6 +408,8 @@ def main():
         "abstraction_ms": abs_ms / K,
         "solve_ms": solve_ms / K,
         "total_ms": (abs_ms + solve_ms) / K,
+        "solve_ms_with_kernel_events": timed_solve_ms / K,
+        "iterations_run": iterations_run,
         "satisfaction_ms_per_iteration": sat_launch_ms,
         "prefix_ms_per_iteration": prefix_ms / iters,
         "bookkeeping_ms_per_iteration": red_ms / iters,

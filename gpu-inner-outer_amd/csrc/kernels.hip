@@ -803,6 +803,35 @@ __device__ inline bool count_test(const SatisfyArgs& A, const uint32_t* PU, uint
     return true;
 }
 
+// The full test of one pair that the row probe could not rule out: two more
+// rows of the unreached bitmask — the box's first and last y rows at its
+// x-range and lower θ, v, where the unreached cells left in a box tend to sit
+// — and, if neither holds one, the 16-corner count (count_test). `counted`
+// counts the pairs that reach the count.
+__device__ __forceinline__ bool pair_satisfied(const SatisfyArgs& A, const uint32_t* PU, uint32_t* rec_plane,
+                                               const uint32_t* zw_plane, const uint32_t* ubits, uint32_t plane,
+                                               uint32_t qs, uint32_t i, uint32_t& counted) {
+    const DevCase& c = A.c;
+    const uint32_t st1 = c.stride[1];
+    const uint32_t sy = qs / c.shape[0];
+    const uint32_t sx = qs - sy * c.shape[0];
+    uint32_t* const slot = rec_plane + rec_offset(c.PQ, qs, i);
+    const uint32_t rj = *slot;
+    if (!(rj & kRecEscape)) {
+        const uint32_t ra = sx + sy * st1 + rec_off0(rj) +
+                            static_cast<uint32_t>(__mul24(rec_off1(rj), static_cast<int>(st1))) + zw_plane[i];
+        const uint32_t rb = ra + static_cast<uint32_t>(__mul24(static_cast<int>(rec_ext1(rj)), static_cast<int>(st1)));
+        const uint64_t wa = ((static_cast<uint64_t>(ubits[(ra >> 5) + 1]) << 32) | ubits[ra >> 5]) >> (ra & 31u);
+        const uint64_t wb = ((static_cast<uint64_t>(ubits[(rb >> 5) + 1]) << 32) | ubits[rb >> 5]) >> (rb & 31u);
+        if (((wa | wb) & ((2ull << rec_ext0(rj)) - 1ull)) != 0) return false;
+    }
+    const uint32_t iw = i / c.n_a;
+    const uint2 b2 = A.box2[(plane % c.shape[2]) * c.n_w + iw];
+    const uint2 b3 = A.box3[(plane / c.shape[2]) * c.n_a + (i - iw * c.n_a)];
+    ++counted;
+    return count_test(A, PU, rj, sx, sy, b2, b3, slot);
+}
+
 // Totals and the stop test of one iteration (kernels.cu:722-766), by one wave.
 // The iteration number lives on the device (ctl->iterations + 1), so a captured
 // sequence of launches replays correctly.
@@ -940,7 +969,9 @@ __device__ __forceinline__ bool chunk_changed(const SatisfyArgs& A, uint4 ch, ui
 // iteration unless the loop has stopped: the body loads the state and the
 // group's chunk flags together and returns on a stopped loop before touching
 // any state.
-template <uint32_t WG, uint32_t IB>
+// kSweep: the sweep_all_pairs schedule, where each lane counts its own pairs
+// in input order instead of through the count queue.
+template <uint32_t WG, uint32_t IB, bool kSweep>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) void satisfy_kernel(SatisfyArgs A,
                                                                                               uint32_t iter) {
     const uint32_t block = blockIdx.x;
@@ -1190,6 +1221,7 @@ __global__ __launch_bounds__(256) void coarse_batch_kernel(const BatchInstance* 
 template <uint32_t WG, uint32_t IB>
 __device__ __forceinline__ void satisfy_in_batch(const SatisfyArgs& A, uint32_t block, uint32_t iter) {
     constexpr bool kChunkFlags = false;  // chunk hulls tested in the body
+    constexpr bool kSweep = false;       // the count queue in both schedules
 #include "satisfy_body.inc"
 }
 
@@ -1244,8 +1276,9 @@ using SatSingle = void (*)(SatisfyArgs, uint32_t);
 using SatBatch = void (*)(const BatchInstance*, const uint32_t*, uint32_t);
 
 template <uint32_t WG>
-SatSingle pick_single(uint32_t ib) {
-    return ib == 4 ? satisfy_kernel<WG, 4> : satisfy_kernel<WG, 8>;
+SatSingle pick_single(uint32_t ib, bool sweep) {
+    if (sweep) return ib == 4 ? satisfy_kernel<WG, 4, true> : satisfy_kernel<WG, 8, true>;
+    return ib == 4 ? satisfy_kernel<WG, 4, false> : satisfy_kernel<WG, 8, false>;
 }
 template <uint32_t WG>
 SatBatch pick_batch(uint32_t ib) {
@@ -1262,7 +1295,10 @@ uint32_t satisfy_blocks(const DevCase& c, uint32_t wg) {
 }
 
 hipError_t launch_satisfy(const SatisfyArgs& A, uint32_t iter, uint32_t wg, uint32_t ib, hipStream_t stream) {
-    const SatSingle f = wg == 128 ? pick_single<128>(ib) : wg == 512 ? pick_single<512>(ib) : pick_single<256>(ib);
+    const bool sw = A.sweep_all != 0;
+    const SatSingle f = wg == 128   ? pick_single<128>(ib, sw)
+                        : wg == 512 ? pick_single<512>(ib, sw)
+                                    : pick_single<256>(ib, sw);
     hipLaunchKernelGGL(f, dim3(satisfy_blocks(A.c, wg)), dim3(wg), 0, stream, A, iter);
     return hipGetLastError();
 }
