@@ -630,10 +630,15 @@ __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
     int hx0 = kHullMaxOffset + 1, hx1 = -kHullMaxOffset - 1, hy0 = kHullMaxOffset + 1, hy1 = -kHullMaxOffset - 1;
     bool any_valid = false, unrepresentable = false;
     uint32_t i = 0;
+    // The θ / v ranges are wave-uniform per input: scalar loads through the
+    // constant address space (a plain load could alias the record stores
+    // below, and would be a vector load waited for on every input).
+    const ConstPtr<uint32_t> box2 = to_const(reinterpret_cast<const uint32_t*>(A.box2 + iz * c.n_w));
+    const ConstPtr<uint32_t> box3 = to_const(reinterpret_cast<const uint32_t*>(A.box3 + iv * c.n_a));
     for (uint32_t iw = 0; iw < c.n_w; ++iw) {
-        const uint2 b2 = A.box2[iz * c.n_w + iw];
+        const uint2 b2 = make_uint2(box2[2 * iw], box2[2 * iw + 1]);
         for (uint32_t ia = 0; ia < c.n_a; ++ia, ++i) {
-            const uint2 b3 = A.box3[iv * c.n_a + ia];
+            const uint2 b3 = make_uint2(box3[2 * ia], box3[2 * ia + 1]);
             uint32_t r = kRecInvalid;
             uint32_t lo[4], hi[4];
             const double2* const inc = A.inc + (static_cast<size_t>(plane) * c.I + i) * c.K;
@@ -1080,14 +1085,17 @@ __global__ __launch_bounds__(256) void satisfy_recompute_kernel(RecomputeArgs R)
         uint32_t found = GSC_INVALID_INPUT;
         bool pending = has;
         uint32_t i = 0;
+        // Wave-uniform per input: scalar loads.
+        const ConstPtr<uint32_t> box2 = to_const(reinterpret_cast<const uint32_t*>(A.box2 + iz * c.n_w));
+        const ConstPtr<uint32_t> box3 = to_const(reinterpret_cast<const uint32_t*>(A.box3 + iv * c.n_a));
         for (uint32_t iw = 0; iw < c.n_w; ++iw) {
-            const uint2 b2 = A.box2[iz * c.n_w + iw];
+            const uint2 b2 = make_uint2(box2[2 * iw], box2[2 * iw + 1]);
             for (uint32_t ia = 0; ia < c.n_a; ++ia, ++i) {
                 if (__ballot(pending) == 0) break;
                 if (lane == 0) slots += 64u;
                 if (!pending) continue;
                 ++tested;
-                const uint2 b3 = A.box3[iv * c.n_a + ia];
+                const uint2 b3 = make_uint2(box3[2 * ia], box3[2 * ia + 1]);
                 const double2* const inc = R.inc + (static_cast<size_t>(plane) * c.I + i) * c.K;
                 const ConstPtr<double> sums = to_const(R.inc_sums + (static_cast<size_t>(plane) * c.I + i) * 4);
                 uint32_t lo[4], hi[4];
