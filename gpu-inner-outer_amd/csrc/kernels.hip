@@ -226,6 +226,9 @@ __global__ __launch_bounds__(256) void unreached_bits_kernel(uint32_t S, const u
 // Pass 2..4: running sum along `axis` (1, 2 or 3); one thread per line (256 per
 // block), lines enumerated with x fastest so neighbouring threads touch
 // neighbouring words.
+// Values of a prefix line loaded together in the axis passes.
+constexpr uint32_t kLineBatch = 16;
+
 __device__ __forceinline__ void prefix_axis_body(const DevCase& c, int axis, uint32_t* __restrict__ P,
                                                  uint32_t block) {
     const uint32_t tid = block * 256u + threadIdx.x;
@@ -242,24 +245,20 @@ __device__ __forceinline__ void prefix_axis_body(const DevCase& c, int axis, uin
     uint32_t off = (x + 1) + (i1 + 1) * c.pstride[o1] + (i2 + 1) * c.pstride[o2] + c.pstride[axis];
     const uint32_t step = c.pstride[axis];
     const uint32_t n = c.shape[axis];
+    // kLineBatch values of the line are loaded before any is stored, so that
+    // one memory round trip covers them (a store could alias a later load of
+    // the same line, which would otherwise keep every load behind it).
     uint32_t acc = 0;
-    uint32_t k = 0;
-    for (; k + 4 <= n; k += 4) {
-        const uint32_t v0 = P[off], v1 = P[off + step], v2 = P[off + 2 * step], v3 = P[off + 3 * step];
-        acc += v0;
-        P[off] = acc;
-        acc += v1;
-        P[off + step] = acc;
-        acc += v2;
-        P[off + 2 * step] = acc;
-        acc += v3;
-        P[off + 3 * step] = acc;
-        off += 4 * step;
-    }
-    for (; k < n; ++k) {
-        acc += P[off];
-        P[off] = acc;
-        off += step;
+    for (uint32_t k = 0; k < n; k += kLineBatch) {
+        uint32_t v[kLineBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kLineBatch; ++j) v[j] = k + j < n ? P[off + j * step] : 0u;
+#pragma unroll
+        for (uint32_t j = 0; j < kLineBatch; ++j) {
+            acc += v[j];
+            if (k + j < n) P[off + j * step] = acc;
+        }
+        off += kLineBatch * step;
     }
 }
 
@@ -381,13 +380,22 @@ __global__ __launch_bounds__(256) void prefix_last_axis_kernel(DevCase c, uint32
     uint32_t coff = 0;
     if (sample)
         coff = (x + kCoarse) / kCoarse + (y + kCoarse) / kCoarse * c.cstride[1] + (z + 1) * c.cstride[2] + c.cstride[3];
+    const uint32_t n = c.shape[3];
     uint32_t acc = 0;
-    for (uint32_t k = 0; k < c.shape[3]; ++k) {
-        acc += P[off];
-        P[off] = acc;
-        if (sample) C[coff] = acc;
-        off += step;
-        coff += c.cstride[3];
+    for (uint32_t k = 0; k < n; k += kLineBatch) {
+        uint32_t v[kLineBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kLineBatch; ++j) v[j] = k + j < n ? P[off + j * step] : 0u;
+#pragma unroll
+        for (uint32_t j = 0; j < kLineBatch; ++j) {
+            acc += v[j];
+            if (k + j < n) {
+                P[off + j * step] = acc;
+                if (sample) C[coff + j * c.cstride[3]] = acc;
+            }
+        }
+        off += kLineBatch * step;
+        coff += kLineBatch * c.cstride[3];
     }
 }
 
