@@ -283,6 +283,57 @@ GSC_HD inline int dim_to_cells_fast(const DevCase& c, int d, double v0, double q
     return reject ? kCellsReject : (inside & clear) ? kCellsOk : kCellsUndecided;
 }
 
+// ---- per-(plane, input) cell offsets ---------------------------------------
+// For a cell centre, v0 = (x0 - lb) / eta is i + 0.5 up to a few roundings,
+// so the lower quotient is i + (0.5 + qa) and the upper i + (0.5 + qb) for
+// every cell i of the plane. When the fractional parts of ua = 0.5 + qa, of
+// ub = 0.5 + qb and of the domain limits relative to them are clear of the
+// filter bound, the reference's cells are i + floor(ua) and i + ceil(ub) - 1
+// (clamped to the grid as dim_to_cells clamps them) and the box leaves the
+// domain exactly for i outside [rej_lo, rej_hi]: the cell's own rounding
+// (v0 against i + 0.5, and the sums ua, ub, t - ua) adds at most 4u qmax +
+// X u / eta + 2u (qmax + |q|) to the bound above, less than twice it, and the
+// bound used is kFilterWiden times it. cell_offsets returns false when any
+// fractional part is too close to an integer; the kernels then take
+// dim_to_cells_fast for that (plane, input).
+struct CellOffsets {
+    int32_t off_lo, off_hi;  // cell offsets of the lower and upper edge
+    int32_t rej_lo, rej_hi;  // cells i < rej_lo or i > rej_hi leave the domain
+};
+constexpr int32_t kOffsetsUndecided = INT32_MIN;  // off_lo of a (plane, input) left to dim_to_cells_fast
+
+GSC_HD inline bool cell_offsets(const DevCase& c, int d, double qa, double qb, CellOffsets* o) {
+    const double eq = c.flt_q[d];
+    const double ua = 0.5 + qa, ub = 0.5 + qb;
+    // Quotients of the domain limits lb - kEps / ub + kEps, as set_cell_filter forms them.
+    const double t_lo = (c.lb_minus_eps[d] - c.lb[d]) * c.inv_eta[d];
+    const double t_hi = (c.ub_plus_eps[d] - c.lb[d]) * c.inv_eta[d];
+    const double A = t_lo - ua, B = t_hi - ub;
+    const double lim = 1073741824.0;  // 2^30: all four stay far inside int32
+    if (!(fabs(ua) < lim && fabs(ub) < lim && fabs(A) < lim && fabs(B) < lim)) return false;
+    const double fa = floor(ua), cb = ceil(ub), fA = floor(A), fB = floor(B);
+    const double r1 = ua - fa, r2 = cb - ub, r3 = A - fA, r4 = B - fB;  // in [0, 1)
+    const bool clear = (r1 > eq) & (r1 < 1.0 - eq) & (r2 > eq) & (r2 < 1.0 - eq) & (r3 > eq) & (r3 < 1.0 - eq) &
+                       (r4 > eq) & (r4 < 1.0 - eq);
+    if (!clear) return false;
+    o->off_lo = static_cast<int32_t>(fa);
+    o->off_hi = static_cast<int32_t>(cb) - 1;
+    o->rej_lo = static_cast<int32_t>(fA) + 1;  // i + ua < t_lo  <=>  i < A  <=>  i <= floor(A)
+    o->rej_hi = static_cast<int32_t>(fB);      // i + ub > t_hi  <=>  i > B  <=>  i >= floor(B) + 1
+    return true;
+}
+
+// The cells of cell i from its (plane, input) offsets: false when the box
+// leaves the domain (dim_to_cells' rejection), else the clamped range.
+GSC_HD inline bool cells_from_offsets(const CellOffsets& o, int32_t i, uint32_t n, uint32_t* lo, uint32_t* hi) {
+    if (i < o.rej_lo || i > o.rej_hi) return false;
+    const int32_t top = static_cast<int32_t>(n) - 1;
+    const int32_t a = i + o.off_lo, b = i + o.off_hi;
+    *lo = static_cast<uint32_t>(a < 0 ? 0 : a > top ? top : a);
+    *hi = static_cast<uint32_t>(b < 0 ? 0 : b > top ? top : b);
+    return true;
+}
+
 // 16-corner inclusion–exclusion on a padded u32 prefix table (prefix_sum.hpp:65-85)
 // for a box with lo <= hi in every dimension. Exact: the true count lies in [0, S].
 GSC_HD inline uint32_t box_count_u32(const uint32_t* __restrict__ P, const uint32_t ps[4], const uint32_t lo[4],

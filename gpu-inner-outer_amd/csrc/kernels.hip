@@ -472,7 +472,8 @@ __global__ __launch_bounds__(256) void increments_kernel(DevCase c, const double
 // rad the plane's radius, (S - rad) / eta and (S + rad) / eta for x and for y.
 // Thread per (plane, input).
 __global__ __launch_bounds__(256) void increment_sums_kernel(DevCase c, const double2* __restrict__ inc,
-                                                              const double* __restrict__ rad, double* __restrict__ sums) {
+                                                              const double* __restrict__ rad, double* __restrict__ sums,
+                                                              int32_t* __restrict__ offs) {
     const uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= static_cast<uint64_t>(c.shape[2]) * c.shape[3] * c.I) return;
     const uint32_t plane = static_cast<uint32_t>(t / c.I);
@@ -485,10 +486,19 @@ __global__ __launch_bounds__(256) void increment_sums_kernel(DevCase c, const do
         sy = sy + v.y;
     }
     const double r0 = rad[2 * iv], r1 = rad[2 * iv + 1];
-    sums[4 * t] = (sx - r0) * c.inv_eta[0];
-    sums[4 * t + 1] = (sx + r0) * c.inv_eta[0];
-    sums[4 * t + 2] = (sy - r1) * c.inv_eta[1];
-    sums[4 * t + 3] = (sy + r1) * c.inv_eta[1];
+    const double q[4] = {(sx - r0) * c.inv_eta[0], (sx + r0) * c.inv_eta[0], (sy - r1) * c.inv_eta[1],
+                         (sy + r1) * c.inv_eta[1]};
+    for (int k = 0; k < 4; ++k) sums[4 * t + k] = q[k];
+    // The same edges as cell offsets shared by every cell of the plane
+    // (cell_offsets), or kOffsetsUndecided.
+    for (int d = 0; d < 2; ++d) {
+        CellOffsets o;
+        if (!cell_offsets(c, d, q[2 * d], q[2 * d + 1], &o)) o = CellOffsets{kOffsetsUndecided, 0, 0, 0};
+        offs[8 * t + 4 * d] = o.off_lo;
+        offs[8 * t + 4 * d + 1] = o.off_hi;
+        offs[8 * t + 4 * d + 2] = o.rej_lo;
+        offs[8 * t + 4 * d + 3] = o.rej_hi;
+    }
 }
 
 // θ/v part of the probe cell per (plane, input): lo2 * stride2 + lo3 * stride3 of
@@ -556,20 +566,32 @@ __device__ __noinline__ ExactCells exact_cells(const DevCase* __restrict__ cg, c
 // Returns false for an invalid pair (abstraction.hpp:187-205). `inc` points at
 // the pair's (plane, input) row of K increments and must be wave-uniform.
 __device__ __forceinline__ bool pair_box(const DevCase& c, const DevCase* cg, const double2* inc, ConstPtr<double> sums,
-                                         uint2 b2, uint2 b3, double x0, double y0, double rad0, double rad1,
-                                         const uint32_t* __restrict__ fail2d, uint32_t lo[4], uint32_t hi[4]) {
+                                         ConstPtr<int32_t> offs, uint2 b2, uint2 b3, uint32_t ix, uint32_t iy, double x0,
+                                         double y0, double rad0, double rad1, const uint32_t* __restrict__ fail2d,
+                                         uint32_t lo[4], uint32_t hi[4]) {
     if (b2.x == kBoxReject || b3.x == kBoxReject) return false;
     lo[2] = b2.x;
     hi[2] = b2.y;
     lo[3] = b3.x;
     hi[3] = b3.y;
-    // Fast path: x0 + (sum of the increments), decided by dim_to_cells_fast when
-    // its error bound leaves no doubt about the reference's cells (x and y are
-    // plain intervals when the wrap axis is θ or v).
+    // Fast paths (x and y are plain intervals when the wrap axis is θ or v):
+    // the (plane, input)'s cell offsets, integer arithmetic only, when they
+    // decided (a wave-uniform test); else x0 + (sum of the increments), decided
+    // by dim_to_cells_fast when its error bound leaves no doubt about the
+    // reference's cells.
     int rx = kCellsUndecided, ry = kCellsUndecided;
     if (c.wrap_dim >= 2) {
-        rx = dim_to_cells_fast(c, 0, (x0 - c.lb[0]) * c.inv_eta[0], sums[0], sums[1], &lo[0], &hi[0]);
-        ry = dim_to_cells_fast(c, 1, (y0 - c.lb[1]) * c.inv_eta[1], sums[2], sums[3], &lo[1], &hi[1]);
+        if (offs[0] != kOffsetsUndecided && offs[4] != kOffsetsUndecided) {
+            const CellOffsets ox{offs[0], offs[1], offs[2], offs[3]};
+            const CellOffsets oy{offs[4], offs[5], offs[6], offs[7]};
+            if (!cells_from_offsets(ox, static_cast<int32_t>(ix), c.shape[0], &lo[0], &hi[0]) ||
+                !cells_from_offsets(oy, static_cast<int32_t>(iy), c.shape[1], &lo[1], &hi[1]))
+                return false;
+            rx = ry = kCellsOk;
+        } else {
+            rx = dim_to_cells_fast(c, 0, (x0 - c.lb[0]) * c.inv_eta[0], sums[0], sums[1], &lo[0], &hi[0]);
+            ry = dim_to_cells_fast(c, 1, (y0 - c.lb[1]) * c.inv_eta[1], sums[2], sums[3], &lo[1], &hi[1]);
+        }
     }
     if (rx == kCellsReject || ry == kCellsReject) return false;
     if (rx == kCellsUndecided || ry == kCellsUndecided) {
@@ -651,9 +673,9 @@ __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
             uint32_t lo[4], hi[4];
             const double2* const inc = A.inc + (static_cast<size_t>(plane) * c.I + i) * c.K;
             const ConstPtr<double> sums = to_const(A.inc_sums + (static_cast<size_t>(plane) * c.I + i) * 4);
-            const bool ok =
-                state_ok && pair_box(c, &kernarg_block<AbstractionArgs>()->c, inc, sums, b2, b3, x0, y0, rad0, rad1,
-                                     A.fail2d, lo, hi);
+            const ConstPtr<int32_t> offs = to_const(A.cell_offs + (static_cast<size_t>(plane) * c.I + i) * 8);
+            const bool ok = state_ok && pair_box(c, &kernarg_block<AbstractionArgs>()->c, inc, sums, offs, b2, b3, ix,
+                                                 iy, x0, y0, rad0, rad1, A.fail2d, lo, hi);
             if (ok) {
                 // Inverted ranges on a non-wrap axis or wrapped x/y ranges take the escape path.
                 const bool regular = lo[0] <= hi[0] && lo[1] <= hi[1] && (lo[2] <= hi[2] || wd == 2) &&
@@ -1106,9 +1128,10 @@ __global__ __launch_bounds__(256) void satisfy_recompute_kernel(RecomputeArgs R)
                 const uint2 b3 = make_uint2(box3[2 * ia], box3[2 * ia + 1]);
                 const double2* const inc = R.inc + (static_cast<size_t>(plane) * c.I + i) * c.K;
                 const ConstPtr<double> sums = to_const(R.inc_sums + (static_cast<size_t>(plane) * c.I + i) * 4);
+                const ConstPtr<int32_t> offs = to_const(R.cell_offs + (static_cast<size_t>(plane) * c.I + i) * 8);
                 uint32_t lo[4], hi[4];
-                if (!state_ok || !pair_box(c, &kernarg_block<RecomputeArgs>()->sat.c, inc, sums, b2, b3, x0, y0, rad0,
-                                           rad1, R.fail2d, lo, hi))
+                if (!state_ok || !pair_box(c, &kernarg_block<RecomputeArgs>()->sat.c, inc, sums, offs, b2, b3, ix, iy,
+                                           x0, y0, rad0, rad1, R.fail2d, lo, hi))
                     continue;
                 const bool regular = lo[0] <= hi[0] && lo[1] <= hi[1] && (lo[2] <= hi[2] || wd == 2) &&
                                      (lo[3] <= hi[3] || wd == 3);

@@ -56,6 +56,7 @@ struct AbstractionArgs {
     DevCase c;
     const double2* inc;          // [plane][input][K] nominal-trajectory increments
     const double* inc_sums;      // [plane][input][4]: (S -/+ rad) / eta for x, then y (increment_sums_kernel)
+    const int32_t* cell_offs;    // [plane][input][8]: CellOffsets for x, then y, or kOffsetsUndecided
     const uint2* box2;           // [n_θ][n_ω] θ cell range (or kBoxReject)
     const uint2* box3;           // [n_v][n_a] v cell range (or kBoxReject)
     const double* rad;           // [n_v][2] x / y box radius
@@ -106,6 +107,7 @@ struct RecomputeArgs {
     SatisfyArgs sat;       // rec is unused
     const double2* inc;    // as AbstractionArgs
     const double* inc_sums;
+    const int32_t* cell_offs;
     const double* rad;
     const uint32_t* fail2d;
 };
@@ -149,7 +151,7 @@ __global__ void prefix_plane_kernel(DevCase c, const uint8_t* a, const uint8_t* 
                                     unsigned long long* bits, uint32_t pitch, const IterCtl* ctl);
 __global__ void prefix_last_axis_kernel(DevCase c, uint32_t* P0, uint32_t* P1, uint32_t* C0, uint32_t* C1,
                                         const IterCtl* ctl);
-__global__ void increment_sums_kernel(DevCase c, const double2* inc, const double* rad, double* sums);
+__global__ void increment_sums_kernel(DevCase c, const double2* inc, const double* rad, double* sums, int32_t* offs);
 // Coarse sample of a prefix table (with ctl: of table k & 1 into coarse k & 1).
 __global__ void coarse_kernel(DevCase c, const uint32_t* P0, const uint32_t* P1, uint32_t* C0, uint32_t* C1,
                               const IterCtl* ctl);

@@ -272,6 +272,7 @@ void gsc_solver::setup() {
     if (n_inc * sizeof(double2) > (4ull << 30)) throw ConfigError("trajectory increment table exceeds 4 GiB");
     inc = alloc<double2>(n_inc);
     inc_sums = alloc<double>(4ull * dc.shape[2] * dc.shape[3] * dc.I);
+    cell_offs = alloc<int32_t>(8ull * dc.shape[2] * dc.shape[3] * dc.I);
     d_obstacles = alloc<gsc_rect4>(obstacles.size());
     d_targets = alloc<gsc_rect4>(targets.size());
 
@@ -317,7 +318,7 @@ void gsc_solver::setup() {
         // Also with no substeps (K = 0): the sums are then zero.
         const uint64_t n_rows = static_cast<uint64_t>(dc.shape[2]) * dc.shape[3] * dc.I;
         hipLaunchKernelGGL(increment_sums_kernel, dim3(blocks_for(n_rows, 256)), dim3(256), 0, stream, dc, inc, rad,
-                           inc_sums);
+                           inc_sums, cell_offs);
         GSC_HIP(hipGetLastError());
     }
     // Masks on the device (the reference builds them in a serial host loop even in
@@ -352,6 +353,7 @@ uint64_t gsc_solver::device_bytes(const gsc_case_shape& sh, uint32_t substeps) {
     b += r(sizeof(OverflowEntry) * (1u << 16)) + r(4) + r(16) + r(sizeof(IterCtl)) + r(sizeof(IterRecord) * kRing);
     b += r(16 * n2 * nw * K) + r(8 * n3 * na * K) + r(8 * n2 * nw) + r(8 * n3 * na) + r(16 * n3);
     b += r(4 * (n0 + 1) * (n1 + 1)) + r(16 * planes * I * K) + r(32 * planes * I) + 2 * r(64 * 64);
+    b += r(32 * planes * I);  // cell offsets
     return b;
 }
 
@@ -397,6 +399,7 @@ void gsc_solver::launch_abstraction() {
     A.c = dc;
     A.inc = inc;
     A.inc_sums = inc_sums;
+    A.cell_offs = cell_offs;
     A.box2 = box2;
     A.box3 = box3;
     A.rad = rad;
@@ -488,6 +491,7 @@ void gsc_solver::solve() {
     RA.sat = A;
     RA.inc = inc;
     RA.inc_sums = inc_sums;
+    RA.cell_offs = cell_offs;
     RA.rad = rad;
     RA.fail2d = fail2d;
     const bool chunk_flags_used = !opts.recompute_abstraction && !opts.sweep_all_pairs && dc.hull_enabled;

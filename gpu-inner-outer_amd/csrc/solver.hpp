@@ -235,6 +235,7 @@ struct gsc_solver {
     double* vt = nullptr;
     double2* inc = nullptr;
     double* inc_sums = nullptr;  // [plane][input][4], see increment_sums_kernel
+    int32_t* cell_offs = nullptr;  // [plane][input][8], see increment_sums_kernel
     unsigned long long* ubits = nullptr;
     uint32_t* probe_zw = nullptr;
     uint2 *box2 = nullptr, *box3 = nullptr;

@@ -9,7 +9,10 @@ cells only when a rigorous error bound keeps every threshold out of reach;
 otherwise they fall back to the exact sum. tests/cpp/cell_filter_check.cpp
 drives both paths through the library's own header on random grids, and on
 coordinates placed onto cell edges and domain limits, and counts any case
-where the filter decides differently from the exact path."""
+where the filter decides differently from the exact path. The same program
+checks the per-(plane, input) cell offsets (cell_offsets in gsc_device.hpp),
+which decide the cells of every cell centre of a plane at once from the
+fractional parts of the shared edge quotients."""
 import os
 import re
 import shutil
@@ -37,10 +40,15 @@ def checker(tmp_path_factory):
 @pytest.mark.parametrize("seed", [1, 2, 3])
 def test_filter_never_disagrees_with_exact_path(checker, seed):
     out = subprocess.run([checker, "2000000", str(seed)], capture_output=True, text=True)
-    m = re.search(r"checked (\d+) decided (\d+) rejected (\d+) mismatches (\d+)", out.stdout)
-    assert m, out.stdout + out.stderr
-    checked, decided, rejected, mismatches = map(int, m.groups())
-    assert out.returncode == 0 and mismatches == 0, out.stdout
-    # Three quarters of the cases sit on a threshold by construction; the filter
-    # still decides most of them, and rejections (domain limits) occur.
-    assert decided > 0.6 * checked and rejected > 0
+    assert out.returncode == 0, out.stdout + out.stderr
+    # Two parts: the per-cell filter (dim_to_cells_fast) and the per-(plane,
+    # input) cell offsets (cell_offsets), each against the exact path.
+    for prefix in ("", "offsets "):
+        m = re.search("^" + prefix + r"checked (\d+) decided (\d+) rejected (\d+) mismatches (\d+)", out.stdout,
+                      re.MULTILINE)
+        assert m, out.stdout + out.stderr
+        checked, decided, rejected, mismatches = map(int, m.groups())
+        assert mismatches == 0, out.stdout
+        # Three quarters of the cases sit on a threshold by construction; both
+        # paths still decide most of them, and rejections (domain limits) occur.
+        assert decided > 0.6 * checked and rejected > 0
