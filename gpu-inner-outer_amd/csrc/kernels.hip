@@ -566,7 +566,7 @@ __device__ __noinline__ ExactCells exact_cells(const DevCase* __restrict__ cg, c
 // Returns false for an invalid pair (abstraction.hpp:187-205). `inc` points at
 // the pair's (plane, input) row of K increments and must be wave-uniform.
 __device__ __forceinline__ bool pair_box(const DevCase& c, const DevCase* cg, const double2* inc, ConstPtr<double> sums,
-                                         ConstPtr<int32_t> offs, uint2 b2, uint2 b3, uint32_t ix, uint32_t iy, double x0,
+                                         const int32_t offs[8], uint2 b2, uint2 b3, uint32_t ix, uint32_t iy, double x0,
                                          double y0, double rad0, double rad1, const uint32_t* __restrict__ fail2d,
                                          uint32_t lo[4], uint32_t hi[4]) {
     if (b2.x == kBoxReject || b3.x == kBoxReject) return false;
@@ -673,7 +673,11 @@ __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
             uint32_t lo[4], hi[4];
             const double2* const inc = A.inc + (static_cast<size_t>(plane) * c.I + i) * c.K;
             const ConstPtr<double> sums = to_const(A.inc_sums + (static_cast<size_t>(plane) * c.I + i) * 4);
-            const ConstPtr<int32_t> offs = to_const(A.cell_offs + (static_cast<size_t>(plane) * c.I + i) * 8);
+            // The (plane, input)'s cell offsets in one scalar load, ahead of the branches that use them.
+            const ConstPtr<int32_t> offs_p = to_const(A.cell_offs + (static_cast<size_t>(plane) * c.I + i) * 8);
+            int32_t offs[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) offs[k] = offs_p[k];
             const bool ok = state_ok && pair_box(c, &kernarg_block<AbstractionArgs>()->c, inc, sums, offs, b2, b3, ix,
                                                  iy, x0, y0, rad0, rad1, A.fail2d, lo, hi);
             if (ok) {
@@ -1128,7 +1132,10 @@ __global__ __launch_bounds__(256) void satisfy_recompute_kernel(RecomputeArgs R)
                 const uint2 b3 = make_uint2(box3[2 * ia], box3[2 * ia + 1]);
                 const double2* const inc = R.inc + (static_cast<size_t>(plane) * c.I + i) * c.K;
                 const ConstPtr<double> sums = to_const(R.inc_sums + (static_cast<size_t>(plane) * c.I + i) * 4);
-                const ConstPtr<int32_t> offs = to_const(R.cell_offs + (static_cast<size_t>(plane) * c.I + i) * 8);
+                const ConstPtr<int32_t> offs_p = to_const(R.cell_offs + (static_cast<size_t>(plane) * c.I + i) * 8);
+                int32_t offs[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) offs[k] = offs_p[k];
                 uint32_t lo[4], hi[4];
                 if (!state_ok || !pair_box(c, &kernarg_block<RecomputeArgs>()->sat.c, inc, sums, offs, b2, b3, ix, iy,
                                            x0, y0, rad0, rad1, R.fail2d, lo, hi))
