@@ -54,6 +54,14 @@ __device__ inline ConstPtr<T> to_const(const T* p) {
     return reinterpret_cast<ConstPtr<T>>(reinterpret_cast<uintptr_t>(p));
 }
 
+// Whether any of bits [row, row + ext] (ext <= 31) is set, given the two
+// 32-bit words w0 = word row / 32 and w1 = the next: a 32-bit funnel shift
+// brings them to bit 0, and shifting left by 31 - ext drops the bits above.
+__device__ __forceinline__ bool bits_any(uint32_t w0, uint32_t w1, uint32_t row, uint32_t ext) {
+    const uint32_t window = __builtin_amdgcn_alignbit(w1, w0, row & 31u);
+    return (window << (31u - ext)) != 0;
+}
+
 __device__ inline bool loop_done(const IterCtl* ctl) {
     return __atomic_load_n(&ctl->done, __ATOMIC_RELAXED) != 0;
 }
@@ -860,9 +868,9 @@ __device__ __forceinline__ bool pair_satisfied(const SatisfyArgs& A, const uint3
         const uint32_t ra = sx + sy * st1 + rec_off0(rj) +
                             static_cast<uint32_t>(__mul24(rec_off1(rj), static_cast<int>(st1))) + zw_plane[i];
         const uint32_t rb = ra + static_cast<uint32_t>(__mul24(static_cast<int>(rec_ext1(rj)), static_cast<int>(st1)));
-        const uint64_t wa = ((static_cast<uint64_t>(ubits[(ra >> 5) + 1]) << 32) | ubits[ra >> 5]) >> (ra & 31u);
-        const uint64_t wb = ((static_cast<uint64_t>(ubits[(rb >> 5) + 1]) << 32) | ubits[rb >> 5]) >> (rb & 31u);
-        if (((wa | wb) & ((2ull << rec_ext0(rj)) - 1ull)) != 0) return false;
+        const uint32_t e = rec_ext0(rj);
+        if (bits_any(ubits[ra >> 5], ubits[(ra >> 5) + 1], ra, e) || bits_any(ubits[rb >> 5], ubits[(rb >> 5) + 1], rb, e))
+            return false;
     }
     const uint32_t iw = i / c.n_a;
     const uint2 b2 = A.box2[(plane % c.shape[2]) * c.n_w + iw];
@@ -1148,9 +1156,7 @@ __global__ __launch_bounds__(256) void satisfy_recompute_kernel(RecomputeArgs R)
                     if (hi[0] - lo[0] <= kExtMax) {
                         const uint32_t row = lo[0] + (lo[1] + ((hi[1] - lo[1]) >> 1)) * c.stride[1] +
                                              lo[2] * c.stride[2] + lo[3] * c.stride[3];
-                        const uint64_t window =
-                            ((static_cast<uint64_t>(ubits[(row >> 5) + 1]) << 32) | ubits[row >> 5]) >> (row & 31u);
-                        if ((window & ((2ull << (hi[0] - lo[0])) - 1ull)) != 0) continue;
+                        if (bits_any(ubits[row >> 5], ubits[(row >> 5) + 1], row, hi[0] - lo[0])) continue;
                     }
                     ++counted;
                     sat = box_count_wrap_u32(PU, c.pstride, wd, c.shape[wd], lo, hi) == 0 &&
