@@ -27,6 +27,13 @@ namespace {
 
 __device__ inline uint32_t lane_id() { return threadIdx.x & 63u; }
 
+// Number of set bits of the wave mask m below this lane (mbcnt: two
+// instructions, against a 64-bit lane mask, an and and two popcounts).
+__device__ inline uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+}
+
 // Inclusive prefix sum across a 64-lane wave.
 __device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
     const uint32_t l = lane_id();
@@ -1099,7 +1106,7 @@ __global__ __launch_bounds__(256) void satisfy_recompute_kernel(RecomputeArgs R)
         uint32_t base = 0;
         for (uint32_t w = 0; w < wave; ++w) base += s_wave_n[w];
         if (eval) {
-            const uint32_t pos = base + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
+            const uint32_t pos = base + lanes_below(m);
             s_q[pos] = q;
             s_flags[pos] = static_cast<uint8_t>(f);
         }
