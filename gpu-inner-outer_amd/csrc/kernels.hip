@@ -200,7 +200,12 @@ __device__ __forceinline__ void prefix_rows_body(const DevCase& c, const uint8_t
     for (uint32_t x0 = 0; x0 < c.shape[0]; x0 += 64) {
         const uint32_t x = x0 + lane_id();
         uint32_t v = 0;
-        if (x < c.shape[0]) v = (a[src + x] != 0 && !(b && b[src + x] != 0)) ? 1u : 0u;
+        if (x < c.shape[0]) {
+            // Both masks loaded unconditionally: a short-circuit would load b only
+            // after a had arrived, two memory round trips instead of one.
+            const uint32_t av = a[src + x], bv = b ? b[src + x] : 0u;
+            v = (av != 0) & (bv == 0) ? 1u : 0u;
+        }
         v = wave_inclusive_scan(v) + carry;
         if (x < c.shape[0]) P[dst + x] = v;
         carry = __shfl(v, 63, 64);
@@ -226,7 +231,12 @@ __device__ __forceinline__ void unreached_bits_body(uint32_t S, const uint8_t* _
                                                     const uint8_t* __restrict__ reach,
                                                     unsigned long long* __restrict__ bits, uint32_t block) {
     const uint32_t s = block * 256u + threadIdx.x;
-    const bool u = s < S && valid[s] != 0 && reach[s] == 0;
+    uint32_t vv = 0, rv = 0;
+    if (s < S) {
+        vv = valid[s];  // both loads issued together (no short-circuit)
+        rv = reach[s];
+    }
+    const bool u = (vv != 0) & (rv == 0);
     const uint64_t b = __ballot(u);
     if (lane_id() == 0 && s < S) bits[s >> 6] = b;
 }
@@ -325,7 +335,12 @@ __global__ __launch_bounds__(kPlaneThreads) void prefix_plane_kernel(DevCase c, 
         const uint32_t k_end = min((base + PQ + 63u) / 64u, (c.S + 63u) / 64u);
         for (uint32_t k = k_begin + wave; k < k_end; k += kW) {
             const uint32_t s = k * 64u + lane;
-            const uint64_t m = __ballot(s < c.S && a[s] != 0 && b[s] == 0);
+            uint32_t av = 0, bv = 0;
+            if (s < c.S) {
+                av = a[s];  // both loads issued together (no short-circuit)
+                bv = b[s];
+            }
+            const uint64_t m = __ballot((av != 0) & (bv == 0));
             if (lane == 0) bits[k] = m;
         }
     }
@@ -336,7 +351,12 @@ __global__ __launch_bounds__(kPlaneThreads) void prefix_plane_kernel(DevCase c, 
         for (uint32_t x0 = 0; x0 < n0; x0 += 64u) {
             const uint32_t x = x0 + lane;
             const uint32_t q = base + y * n0 + x;
-            uint32_t v = (x < n0 && a[q] != 0 && !(b && b[q] != 0)) ? 1u : 0u;
+            uint32_t av = 0, bv = 0;
+            if (x < n0) {
+                av = a[q];  // both loads issued together (no short-circuit)
+                bv = b ? b[q] : 0u;
+            }
+            uint32_t v = (av != 0) & (bv == 0) ? 1u : 0u;
             v = wave_inclusive_scan(v) + carry;
             if (x < n0) sm[y * pitch + (x ^ sw)] = v;
             carry = __shfl(v, 63, 64);
