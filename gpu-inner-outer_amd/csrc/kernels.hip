@@ -35,14 +35,23 @@ __device__ inline uint32_t lanes_below(uint64_t m) {
 }
 
 // Inclusive prefix sum across a 64-lane wave.
+// DPP steps (data moves between lanes inside the VALU, no LDS round trip):
+// shifts by 1, 2, 4, 8 within each 16-lane row (lanes shifted in from outside
+// the row read 0), then row 0's and row 2's last lane broadcast into rows 1
+// and 3, then lane 31 into rows 2 and 3. Rows a step leaves out keep 0 to add.
 __device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
-    const uint32_t l = lane_id();
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t n = __shfl_up(v, off, 64);
-        if (l >= static_cast<uint32_t>(off)) v += n;
-    }
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x111, 0xf, 0xf, true));  // row_shr:1
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x112, 0xf, 0xf, true));  // row_shr:2
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x114, 0xf, 0xf, true));  // row_shr:4
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x118, 0xf, 0xf, true));  // row_shr:8
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x142, 0xa, 0xf, false));  // row_bcast:15
+    v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x143, 0xc, 0xf, false));  // row_bcast:31
     return v;
+}
+
+// Value of lane 63 (wave-uniform).
+__device__ inline uint32_t last_lane(uint32_t v) {
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
 }
 
 __device__ inline uint64_t wave_sum_u64(uint64_t v) {
@@ -208,7 +217,7 @@ __device__ __forceinline__ void prefix_rows_body(const DevCase& c, const uint8_t
         }
         v = wave_inclusive_scan(v) + carry;
         if (x < c.shape[0]) P[dst + x] = v;
-        carry = __shfl(v, 63, 64);
+        carry = last_lane(v);
     }
 }
 
@@ -359,7 +368,7 @@ __global__ __launch_bounds__(kPlaneThreads) void prefix_plane_kernel(DevCase c, 
             uint32_t v = (av != 0) & (bv == 0) ? 1u : 0u;
             v = wave_inclusive_scan(v) + carry;
             if (x < n0) sm[y * pitch + (x ^ sw)] = v;
-            carry = __shfl(v, 63, 64);
+            carry = last_lane(v);
         }
     }
     __syncthreads();
@@ -372,7 +381,7 @@ __global__ __launch_bounds__(kPlaneThreads) void prefix_plane_kernel(DevCase c, 
             uint32_t v = y < n1 ? sm[at] : 0u;
             v = wave_inclusive_scan(v) + carry;
             if (y < n1) sm[at] = v;
-            carry = __shfl(v, 63, 64);
+            carry = last_lane(v);
         }
     }
     __syncthreads();
