@@ -600,6 +600,29 @@ __device__ __noinline__ ExactCells exact_cells(const DevCase* __restrict__ cg, c
     return e;
 }
 
+// The x/y cells of a pair by the per-cell filter (dim_to_cells_fast), and by
+// the exact step-by-step sum where the filter cannot decide: out of line, as
+// the (plane, input) cell offsets decide almost every pair, and the case is
+// read from the argument block (`cg`) rather than held in registers.
+__device__ __noinline__ ExactCells filtered_cells(const DevCase* __restrict__ cg, const double2* __restrict__ inc,
+                                                  const double* __restrict__ sums, double x0, double y0, double rad0,
+                                                  double rad1) {
+    const DevCase& c = *cg;
+    ExactCells e{0, 0, 0, 0, 0};
+    int rx = kCellsUndecided, ry = kCellsUndecided;
+    if (c.wrap_dim >= 2) {
+        rx = dim_to_cells_fast(c, 0, (x0 - c.lb[0]) * c.inv_eta[0], sums[0], sums[1], &e.lo0, &e.hi0);
+        ry = dim_to_cells_fast(c, 1, (y0 - c.lb[1]) * c.inv_eta[1], sums[2], sums[3], &e.lo1, &e.hi1);
+    }
+    if (rx == kCellsReject || ry == kCellsReject) {
+        e.ok = 0;
+        return e;
+    }
+    if (rx == kCellsUndecided || ry == kCellsUndecided) return exact_cells(cg, inc, x0, y0, rad0, rad1);
+    e.ok = 1;
+    return e;
+}
+
 // Successor cell box of one pair, the reference's arithmetic in the reference's order:
 //   centre           grid.hpp:42 (x0, y0 passed in)
 //   nominal x, y     model.hpp:35-42: x += dt*(v_k*cos θ_k), y += dt*(v_k*sin θ_k),
@@ -609,7 +632,7 @@ __device__ __noinline__ ExactCells exact_cells(const DevCase* __restrict__ cg, c
 //   constraint scan  abstraction.hpp:110-159 as a 2D prefix query over x,y fails
 // Returns false for an invalid pair (abstraction.hpp:187-205). `inc` points at
 // the pair's (plane, input) row of K increments and must be wave-uniform.
-__device__ __forceinline__ bool pair_box(const DevCase& c, const DevCase* cg, const double2* inc, ConstPtr<double> sums,
+__device__ __forceinline__ bool pair_box(const DevCase& c, const DevCase* cg, const double2* inc, const double* sums,
                                          const int32_t offs[8], uint2 b2, uint2 b3, uint32_t ix, uint32_t iy, double x0,
                                          double y0, double rad0, double rad1, const uint32_t* __restrict__ fail2d,
                                          uint32_t lo[4], uint32_t hi[4]) {
@@ -618,28 +641,17 @@ __device__ __forceinline__ bool pair_box(const DevCase& c, const DevCase* cg, co
     hi[2] = b2.y;
     lo[3] = b3.x;
     hi[3] = b3.y;
-    // Fast paths (x and y are plain intervals when the wrap axis is θ or v):
-    // the (plane, input)'s cell offsets, integer arithmetic only, when they
-    // decided (a wave-uniform test); else x0 + (sum of the increments), decided
-    // by dim_to_cells_fast when its error bound leaves no doubt about the
-    // reference's cells.
-    int rx = kCellsUndecided, ry = kCellsUndecided;
-    if (c.wrap_dim >= 2) {
-        if (offs[0] != kOffsetsUndecided && offs[4] != kOffsetsUndecided) {
-            const CellOffsets ox{offs[0], offs[1], offs[2], offs[3]};
-            const CellOffsets oy{offs[4], offs[5], offs[6], offs[7]};
-            if (!cells_from_offsets(ox, static_cast<int32_t>(ix), c.shape[0], &lo[0], &hi[0]) ||
-                !cells_from_offsets(oy, static_cast<int32_t>(iy), c.shape[1], &lo[1], &hi[1]))
-                return false;
-            rx = ry = kCellsOk;
-        } else {
-            rx = dim_to_cells_fast(c, 0, (x0 - c.lb[0]) * c.inv_eta[0], sums[0], sums[1], &lo[0], &hi[0]);
-            ry = dim_to_cells_fast(c, 1, (y0 - c.lb[1]) * c.inv_eta[1], sums[2], sums[3], &lo[1], &hi[1]);
-        }
-    }
-    if (rx == kCellsReject || ry == kCellsReject) return false;
-    if (rx == kCellsUndecided || ry == kCellsUndecided) {
-        const ExactCells e = exact_cells(cg, inc, x0, y0, rad0, rad1);
+    // Fast path (x and y are plain intervals when the wrap axis is θ or v): the
+    // (plane, input)'s cell offsets, integer arithmetic only, when they decided
+    // (a wave-uniform test); else filtered_cells.
+    if (c.wrap_dim >= 2 && offs[0] != kOffsetsUndecided && offs[4] != kOffsetsUndecided) {
+        const CellOffsets ox{offs[0], offs[1], offs[2], offs[3]};
+        const CellOffsets oy{offs[4], offs[5], offs[6], offs[7]};
+        if (!cells_from_offsets(ox, static_cast<int32_t>(ix), c.shape[0], &lo[0], &hi[0]) ||
+            !cells_from_offsets(oy, static_cast<int32_t>(iy), c.shape[1], &lo[1], &hi[1]))
+            return false;
+    } else {
+        const ExactCells e = filtered_cells(cg, inc, sums, x0, y0, rad0, rad1);
         if (!e.ok) return false;
         lo[0] = e.lo0;
         hi[0] = e.hi0;
@@ -672,6 +684,29 @@ __device__ __forceinline__ bool state_constraint_ok(const DevCase& c, const uint
     const uint32_t f = fail2d[(iy + 1) * fpitch + ix + 1] - fail2d[(iy + 1) * fpitch + ix] -
                        fail2d[iy * fpitch + ix + 1] + fail2d[iy * fpitch + ix];
     return f == 0;
+}
+
+// A box that does not fit the compact record: an overflow entry with its flat
+// corners and the dead bit settled now; the record is kRecEscape | entry, or
+// kRecInvalid when the overflow table is full (the host sees the count and
+// reruns).
+__device__ __forceinline__ uint32_t escape_record(const AbstractionArgs* __restrict__ Ag, uint32_t lo0, uint32_t lo1,
+                                               uint32_t lo2, uint32_t lo3, uint32_t hi0, uint32_t hi1, uint32_t hi2,
+                                               uint32_t hi3) {
+    const AbstractionArgs& A = *Ag;
+    const DevCase& c = A.c;
+    const uint32_t lo[4] = {lo0, lo1, lo2, lo3}, hi[4] = {hi0, hi1, hi2, hi3};
+    const int wd = c.wrap_dim;
+    const int64_t vc = box_count_i64(A.prefix_valid, c.pstride, wd, c.shape[wd], lo, hi);
+    const uint32_t e = atomicAdd(A.overflow_count, 1u);
+    if (e >= A.overflow_capacity) return kRecInvalid;
+    OverflowEntry o;
+    o.min_flat = lo[0] * c.stride[0] + lo[1] * c.stride[1] + lo[2] * c.stride[2] + lo[3] * c.stride[3];
+    o.max_flat = hi[0] * c.stride[0] + hi[1] * c.stride[1] + hi[2] * c.stride[2] + hi[3] * c.stride[3];
+    o.dead = vc == 0 ? 1u : 0u;
+    o.reserved = 0;
+    A.overflow[e] = o;
+    return kRecEscape | e;
 }
 
 // Abstraction: every pair's box, packed into its record (see gsc_device.hpp).
@@ -716,7 +751,7 @@ __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
             uint32_t r = kRecInvalid;
             uint32_t lo[4], hi[4];
             const double2* const inc = A.inc + (static_cast<size_t>(plane) * c.I + i) * c.K;
-            const ConstPtr<double> sums = to_const(A.inc_sums + (static_cast<size_t>(plane) * c.I + i) * 4);
+            const double* const sums = A.inc_sums + (static_cast<size_t>(plane) * c.I + i) * 4;
             // The (plane, input)'s cell offsets in one scalar load, ahead of the branches that use them.
             const ConstPtr<int32_t> offs_p = to_const(A.cell_offs + (static_cast<size_t>(plane) * c.I + i) * 8);
             int32_t offs[8];
@@ -746,19 +781,8 @@ __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
                 } else if (fits) {
                     r = rec_pack(off0, hi[0] - lo[0], off1, hi[1] - lo[1], false);
                 } else {
-                    const int64_t vc = box_count_i64(A.prefix_valid, c.pstride, wd, c.shape[wd], lo, hi);
-                    const uint32_t e = atomicAdd(A.overflow_count, 1u);
-                    if (e < A.overflow_capacity) {
-                        OverflowEntry o;
-                        o.min_flat = lo[0] * c.stride[0] + lo[1] * c.stride[1] + lo[2] * c.stride[2] + lo[3] * c.stride[3];
-                        o.max_flat = hi[0] * c.stride[0] + hi[1] * c.stride[1] + hi[2] * c.stride[2] + hi[3] * c.stride[3];
-                        o.dead = vc == 0 ? 1u : 0u;
-                        o.reserved = 0;
-                        A.overflow[e] = o;
-                        r = kRecEscape | e;
-                    } else {
-                        r = kRecInvalid;  // capacity exceeded: the host sees the count and reruns
-                    }
+                    r = escape_record(kernarg_block<AbstractionArgs>(), lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2],
+                                      hi[3]);
                 }
             }
             if (live && store) rec_plane[rec_offset(c.PQ, q, i)] = r;
@@ -1175,7 +1199,7 @@ __global__ __launch_bounds__(256) void satisfy_recompute_kernel(RecomputeArgs R)
                 ++tested;
                 const uint2 b3 = make_uint2(box3[2 * ia], box3[2 * ia + 1]);
                 const double2* const inc = R.inc + (static_cast<size_t>(plane) * c.I + i) * c.K;
-                const ConstPtr<double> sums = to_const(R.inc_sums + (static_cast<size_t>(plane) * c.I + i) * 4);
+                const double* const sums = R.inc_sums + (static_cast<size_t>(plane) * c.I + i) * 4;
                 const ConstPtr<int32_t> offs_p = to_const(R.cell_offs + (static_cast<size_t>(plane) * c.I + i) * 8);
                 int32_t offs[8];
 #pragma unroll
