@@ -738,15 +738,25 @@ __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
     // Running hull of the state's valid boxes (see gsc_device.hpp).
     int hx0 = kHullMaxOffset + 1, hx1 = -kHullMaxOffset - 1, hy0 = kHullMaxOffset + 1, hy1 = -kHullMaxOffset - 1;
     bool any_valid = false, unrepresentable = false;
-    uint32_t i = 0;
     // The θ / v ranges are wave-uniform per input: scalar loads through the
     // constant address space (a plain load could alias the record stores
     // below, and would be a vector load waited for on every input).
     const ConstPtr<uint32_t> box2 = to_const(reinterpret_cast<const uint32_t*>(A.box2 + iz * c.n_w));
     const ConstPtr<uint32_t> box3 = to_const(reinterpret_cast<const uint32_t*>(A.box3 + iv * c.n_a));
-    for (uint32_t iw = 0; iw < c.n_w; ++iw) {
-        const uint2 b2 = make_uint2(box2[2 * iw], box2[2 * iw + 1]);
-        for (uint32_t ia = 0; ia < c.n_a; ++ia, ++i) {
+    // Inputs four at a time: the four pairs' scalar loads and arithmetic
+    // interleave, and their records leave as one 16-byte store per lane (half
+    // of the lane's 8-input record block).
+    constexpr uint32_t kAbsGroup = 4;
+    static_assert(kRecBlock % kAbsGroup == 0, "a group stays inside one record block");
+    for (uint32_t i0 = 0; i0 < c.I; i0 += kAbsGroup) {
+        uint32_t rr[kAbsGroup];
+#pragma unroll
+        for (uint32_t j = 0; j < kAbsGroup; ++j) {
+            const uint32_t i = i0 + j;
+            rr[j] = kRecInvalid;
+            if (i >= c.I) continue;
+            const uint32_t iw = i / c.n_a, ia = i - iw * c.n_a;
+            const uint2 b2 = make_uint2(box2[2 * iw], box2[2 * iw + 1]);
             const uint2 b3 = make_uint2(box3[2 * ia], box3[2 * ia + 1]);
             uint32_t r = kRecInvalid;
             uint32_t lo[4], hi[4];
@@ -785,8 +795,10 @@ __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
                                       hi[3]);
                 }
             }
-            if (live && store) rec_plane[rec_offset(c.PQ, q, i)] = r;
+            rr[j] = r;
         }
+        if (live && store)
+            *reinterpret_cast<uint4*>(rec_plane + rec_offset(c.PQ, q, i0)) = make_uint4(rr[0], rr[1], rr[2], rr[3]);
     }
     if (live && A.hull) {
         uint32_t h;
