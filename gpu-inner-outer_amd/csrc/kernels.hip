@@ -733,8 +733,6 @@ __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
     const int wd = c.wrap_dim;
     uint32_t* const rec_plane = A.rec + static_cast<uint64_t>(plane) * c.I_pad * c.PQ;
     const bool store = A.rec != nullptr;  // false: state hulls only
-    if (live && store)
-        for (uint32_t i = c.I; i < c.I_pad; ++i) rec_plane[rec_offset(c.PQ, q, i)] = kRecInvalid;
     // Running hull of the state's valid boxes (see gsc_device.hpp).
     int hx0 = kHullMaxOffset + 1, hx1 = -kHullMaxOffset - 1, hy0 = kHullMaxOffset + 1, hy1 = -kHullMaxOffset - 1;
     bool any_valid = false, unrepresentable = false;
@@ -743,11 +741,11 @@ __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
     // below, and would be a vector load waited for on every input).
     const ConstPtr<uint32_t> box2 = to_const(reinterpret_cast<const uint32_t*>(A.box2 + iz * c.n_w));
     const ConstPtr<uint32_t> box3 = to_const(reinterpret_cast<const uint32_t*>(A.box3 + iv * c.n_a));
-    // Inputs four at a time: the four pairs' scalar loads and arithmetic
-    // interleave, and their records leave as one 16-byte store per lane (half
-    // of the lane's 8-input record block).
-    constexpr uint32_t kAbsGroup = 4;
-    static_assert(kRecBlock % kAbsGroup == 0, "a group stays inside one record block");
+    // Inputs a record block (8) at a time: the pairs' scalar loads and
+    // arithmetic interleave, and their records leave as two 16-byte stores per
+    // lane, padding inputs (I to I_pad) included as kRecInvalid.
+    constexpr uint32_t kAbsGroup = 8;
+    static_assert(kRecBlock == kAbsGroup, "a group is one record block");
     for (uint32_t i0 = 0; i0 < c.I; i0 += kAbsGroup) {
         uint32_t rr[kAbsGroup];
 #pragma unroll
@@ -797,8 +795,11 @@ __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
             }
             rr[j] = r;
         }
-        if (live && store)
-            *reinterpret_cast<uint4*>(rec_plane + rec_offset(c.PQ, q, i0)) = make_uint4(rr[0], rr[1], rr[2], rr[3]);
+        if (live && store) {
+            uint4* const dst = reinterpret_cast<uint4*>(rec_plane + rec_offset(c.PQ, q, i0));
+            dst[0] = make_uint4(rr[0], rr[1], rr[2], rr[3]);
+            dst[1] = make_uint4(rr[4], rr[5], rr[6], rr[7]);
+        }
     }
     if (live && A.hull) {
         uint32_t h;
