@@ -1124,9 +1124,6 @@ __global__ __launch_bounds__(256) void chunk_flags_kernel(SatisfyArgs A, uint8_t
 // time per lane, lanes leave at their first satisfied input.
 __global__ __launch_bounds__(256) void satisfy_recompute_kernel(RecomputeArgs R) {
     const SatisfyArgs& A = R.sat;
-    const uint64_t st = loop_state(A.ctl);
-    if (state_done(st)) return;
-    const uint32_t iter = state_next_iteration(st);
     const DevCase& c = A.c;
     constexpr uint32_t kUse = 1, kCand = 2, kReached = 4, kCertified = 8;
     __shared__ uint32_t s_q[256];
@@ -1138,15 +1135,18 @@ __global__ __launch_bounds__(256) void satisfy_recompute_kernel(RecomputeArgs R)
     const uint32_t gpp = (PQ + 255u) / 256u;
     const uint32_t plane = blockIdx.x / gpp;
     const uint32_t g0 = (blockIdx.x - plane * gpp) * 256u;
+    const bool skip_unchanged = !A.sweep_all && c.hull_enabled;
+    // The loop state and the group's four chunk flags (chunk_flags_kernel, one
+    // aligned word) together; a group with no changed chunk ends here.
+    const uint64_t st = loop_state(A.ctl);
+    uint32_t gf = 0xFFFFFFFFu;
+    if (skip_unchanged) gf = *reinterpret_cast<const uint32_t*>(A.chunk_flags + plane * chunk_flag_pitch(c) + (g0 >> 6));
+    if (state_done(st) || gf == 0) return;
+    const uint32_t iter = state_next_iteration(st);
     const uint32_t* const PU = A.prefix_unreached[iter & 1u];
     // The first iteration compares with "nothing reached yet" (the valid table).
     const uint32_t* const Pprev = iter == 1 ? A.prefix_valid : A.prefix_unreached[(iter - 1u) & 1u];
-    const uint32_t* const CU = A.coarse_unreached[iter & 1u];
-    const uint32_t* const Cprev = iter == 1 ? A.coarse_valid : A.coarse_unreached[(iter - 1u) & 1u];
-    const bool skip_unchanged = !A.sweep_all && c.hull_enabled;
-    bool chunk_live = true;
-    if (skip_unchanged && g0 + 64u * wave < PQ)
-        chunk_live = chunk_changed(A, A.chunk_hull[plane * c.chunks_per_plane + (g0 >> 6) + wave], plane, CU, Cprev);
+    const bool chunk_live = ((gf >> (8u * wave)) & 0xFFu) != 0;
     {
         const uint32_t q = g0 + threadIdx.x;
         const uint32_t s = plane * PQ + q;

@@ -494,7 +494,7 @@ void gsc_solver::solve() {
     RA.cell_offs = cell_offs;
     RA.rad = rad;
     RA.fail2d = fail2d;
-    const bool chunk_flags_used = !opts.recompute_abstraction && !opts.sweep_all_pairs && dc.hull_enabled;
+    const bool chunk_flags_used = !opts.sweep_all_pairs && dc.hull_enabled;
     const uint32_t max_it = cfg.max_iterations;
     uint32_t done = 0;
     uint32_t finished = 0;
@@ -527,8 +527,8 @@ void gsc_solver::solve() {
                                    reach, ubits, ctl);
                 GSC_HIP(hipGetLastError());
             }
-            // Which 64-cell chunks saw a change in their hull, for the stored-record
-            // satisfaction's frontier skipping.
+            // Which 64-cell chunks saw a change in their hull, for the
+            // satisfaction's frontier skipping (both abstraction modes).
             if (chunk_flags_used) GSC_HIP(launch_chunk_flags(A, chunk_flags, it, stream));
             if (timed) GSC_HIP(hipEventRecord(ev[1], stream));
             if (opts.recompute_abstraction) {
