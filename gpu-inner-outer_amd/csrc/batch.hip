@@ -20,7 +20,7 @@
 namespace gsc_amd {
 namespace {
 
-enum Stage : uint32_t { kSeed = 0, kRows, kAxis1, kAxis2, kAxis3, kCoarseStage, kBits, kSat, kStages };
+enum Stage : uint32_t { kSeed = 0, kRows, kAxis1, kAxis2, kAxis3, kCoarseStage, kBits, kChunk, kSat, kStages };
 
 bool options_valid(const gsc_batch_options& o) {
     return satisfy_variant_supported(static_cast<uint32_t>(o.tile_wg), static_cast<uint32_t>(o.tile_ib)) &&
@@ -166,6 +166,7 @@ void gsc_batch::plan() {
         }
         blocks[kCoarseStage] = (s.coarse_count + 255) / 256;
         blocks[kBits] = (c.S + 255) / 256;
+        blocks[kChunk] = (c.n_tasks + 255) / 256;
         blocks[kSat] = satisfy_blocks(c, static_cast<uint32_t>(opts.tile_wg));
         for (uint32_t st = 0; st < kStages; ++st) {
             starts[st * (N + 1) + p] = static_cast<uint32_t>(total[st]);
@@ -181,6 +182,7 @@ void gsc_batch::plan() {
         b.sat.coarse_valid = s.coarse_valid;
         b.sat.hull = s.hull;
         b.sat.chunk_hull = s.chunk_hull;
+        b.sat.chunk_flags = s.chunk_flags;
         b.sat.hull2 = s.hull2;
         b.sat.hull3 = s.hull3;
         b.sat.prefix_valid = s.prefix_valid;
@@ -234,6 +236,8 @@ void gsc_batch::enqueue_iterations(uint32_t iterations) {
                            st + kCoarseStage * (N + 1), N);
         hipLaunchKernelGGL(unreached_bits_batch_kernel, dim3(grid[kBits]), dim3(256), 0, stream, d_inst,
                            st + kBits * (N + 1), N);
+        hipLaunchKernelGGL(chunk_flags_batch_kernel, dim3(grid[kChunk]), dim3(256), 0, stream, d_inst,
+                           st + kChunk * (N + 1), N);
         GSC_HIP(launch_satisfy_batch(d_inst, st + kSat * (N + 1), N, grid[kSat], static_cast<uint32_t>(opts.tile_wg),
                                      static_cast<uint32_t>(opts.tile_ib), stream));
         hipLaunchKernelGGL(finish_batch_kernel, dim3(N), dim3(64), 0, stream, d_inst, N, d_status);

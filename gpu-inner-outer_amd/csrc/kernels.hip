@@ -1104,17 +1104,21 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) void sa
 // state whose unreached count changed since the previous iteration. The
 // satisfaction workgroups read their chunks' flags first and end at once when
 // none is set, without reading their states. Padding bytes keep their zeros.
-__global__ __launch_bounds__(256) void chunk_flags_kernel(SatisfyArgs A, uint8_t* __restrict__ flags,
-                                                          uint32_t iter) {
-    if (loop_done(A.ctl)) return;
+__device__ __forceinline__ void chunk_flags_body(const SatisfyArgs& A, uint8_t* __restrict__ flags, uint32_t iter,
+                                                 uint32_t k) {
     const DevCase& c = A.c;
-    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
     if (k >= c.n_tasks) return;
     const uint32_t plane = k / c.chunks_per_plane;
     const uint32_t j = k - plane * c.chunks_per_plane;
     const uint32_t* const CU = A.coarse_unreached[iter & 1u];
     const uint32_t* const Cprev = iter == 1 ? A.coarse_valid : A.coarse_unreached[(iter - 1u) & 1u];
     flags[plane * chunk_flag_pitch(c) + j] = chunk_changed(A, A.chunk_hull[k], plane, CU, Cprev) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void chunk_flags_kernel(SatisfyArgs A, uint8_t* __restrict__ flags,
+                                                          uint32_t iter) {
+    if (loop_done(A.ctl)) return;
+    chunk_flags_body(A, flags, iter, blockIdx.x * 256u + threadIdx.x);
 }
 
 // Pair satisfaction + first-input reduction with the successor boxes recomputed
@@ -1341,12 +1345,24 @@ __global__ __launch_bounds__(256) void coarse_batch_kernel(const BatchInstance* 
                 (blockIdx.x - starts[k]) * 256u + threadIdx.x);
 }
 
+// Chunk flags of every instance (chunk_flags_kernel), into each case's own table.
+__global__ __launch_bounds__(256) void chunk_flags_batch_kernel(const BatchInstance* __restrict__ B,
+                                                                const uint32_t* __restrict__ starts, uint32_t n) {
+    const uint32_t k = find_instance(starts, n, blockIdx.x);
+    const BatchInstance& I = B[k];
+    const uint64_t st = loop_state(I.sat.ctl);
+    if (state_done(st)) return;
+    if (I.sat.sweep_all || !I.sat.c.hull_enabled) return;  // no frontier skipping: the flags are not read
+    chunk_flags_body(I.sat, const_cast<uint8_t*>(I.sat.chunk_flags), state_next_iteration(st),
+                     (blockIdx.x - starts[k]) * 256u + threadIdx.x);
+}
+
 // The batched kernel reads its case's arguments from global memory; there the
 // function boundary gives the better register allocation (74 vs 97 VGPRs).
 template <uint32_t WG, uint32_t IB>
 __device__ __forceinline__ void satisfy_in_batch(const SatisfyArgs& A, uint32_t block, uint32_t iter) {
-    constexpr bool kChunkFlags = false;  // chunk hulls tested in the body
-    constexpr bool kSweep = false;       // the count queue in both schedules
+    constexpr bool kChunkFlags = true;  // chunk_flags_batch_kernel ran this iteration
+    constexpr bool kSweep = false;      // the count queue in both schedules
 #include "satisfy_body.inc"
 }
 

@@ -183,6 +183,7 @@ __global__ void prefix_axis_batch_kernel(const BatchInstance* B, const uint32_t*
 __global__ void unreached_bits_batch_kernel(const BatchInstance* B, const uint32_t* starts, uint32_t n);
 __global__ void finish_batch_kernel(const BatchInstance* B, uint32_t n, BatchStatus* status);
 __global__ void seed_batch_kernel(const BatchInstance* B, const uint32_t* starts, uint32_t n);
+__global__ void chunk_flags_batch_kernel(const BatchInstance* B, const uint32_t* starts, uint32_t n);
 hipError_t launch_satisfy_batch(const BatchInstance* B, const uint32_t* starts, uint32_t n, uint32_t blocks,
                                 uint32_t wg, uint32_t ib, hipStream_t stream);
 
