@@ -20,7 +20,7 @@
 namespace gsc_amd {
 namespace {
 
-enum Stage : uint32_t { kSeed = 0, kRows, kAxis1, kAxis2, kAxis3, kCoarseStage, kBits, kChunk, kSat, kStages };
+enum Stage : uint32_t { kSeed = 0, kRows, kAxis1, kAxis2, kAxis3, kCoarseStage, kBits, kChunk, kSat, kPlane, kLast, kStages };
 
 bool options_valid(const gsc_batch_options& o) {
     return satisfy_variant_supported(static_cast<uint32_t>(o.tile_wg), static_cast<uint32_t>(o.tile_ib)) &&
@@ -46,6 +46,10 @@ struct gsc_batch {
     BatchStatus* h_status = nullptr;  // pinned
     IterRecord* h_records = nullptr;  // pinned
     uint32_t grid[kStages] = {};
+    // Fused prefix build (plane kernel, θ pass, v pass with the coarse sample)
+    // when every case's plane fits in LDS; its dynamic LDS bytes.
+    bool fused_prefix = false;
+    uint32_t plane_lds = 0;
     std::vector<void*> device_allocs;
 
     hipGraph_t graph = nullptr;
@@ -153,6 +157,8 @@ void gsc_batch::plan() {
     std::vector<BatchInstance> h_inst(N);
     std::vector<uint32_t> starts(static_cast<size_t>(kStages) * (N + 1), 0);
     uint64_t total[kStages] = {};
+    bool fused = true;
+    uint64_t lds = 0;
     for (uint32_t p = 0; p < N; ++p) {
         const gsc_solver& s = *cases[order[p]];
         const DevCase& c = s.dc;
@@ -167,6 +173,8 @@ void gsc_batch::plan() {
         blocks[kCoarseStage] = (s.coarse_count + 255) / 256;
         blocks[kBits] = (c.S + 255) / 256;
         blocks[kChunk] = (c.n_tasks + 255) / 256;
+        blocks[kPlane] = static_cast<uint64_t>(c.shape[2]) * c.shape[3];
+        blocks[kLast] = (static_cast<uint64_t>(c.shape[0]) * c.shape[1] * c.shape[2] + 255) / 256;
         blocks[kSat] = satisfy_blocks(c, static_cast<uint32_t>(opts.tile_wg));
         for (uint32_t st = 0; st < kStages; ++st) {
             starts[st * (N + 1) + p] = static_cast<uint32_t>(total[st]);
@@ -183,6 +191,9 @@ void gsc_batch::plan() {
         b.sat.hull = s.hull;
         b.sat.chunk_hull = s.chunk_hull;
         b.sat.chunk_flags = s.chunk_flags;
+        b.plane_pitch = s.plane_pitch;
+        fused = fused && s.plane_pitch != 0;
+        lds = std::max<uint64_t>(lds, static_cast<uint64_t>(s.plane_pitch) * c.shape[1] * sizeof(uint32_t));
         b.sat.hull2 = s.hull2;
         b.sat.hull3 = s.hull3;
         b.sat.prefix_valid = s.prefix_valid;
@@ -208,6 +219,8 @@ void gsc_batch::plan() {
         b.candidate_count = s.candidate_count;
         b.max_iterations = s.cfg.max_iterations;
     }
+    fused_prefix = fused && N > 0;
+    plane_lds = static_cast<uint32_t>(lds);
     for (uint32_t st = 0; st < kStages; ++st) {
         starts[st * (N + 1) + N] = static_cast<uint32_t>(total[st]);
         grid[st] = static_cast<uint32_t>(total[st]);
@@ -225,17 +238,26 @@ void gsc_batch::enqueue_iterations(uint32_t iterations) {
     const uint32_t N = n();
     const uint32_t* st = d_starts;
     for (uint32_t it = 0; it < iterations; ++it) {
-        hipLaunchKernelGGL(prefix_rows_batch_kernel, dim3(grid[kRows]), dim3(256), 0, stream, d_inst,
-                           st + kRows * (N + 1), N);
-        for (int axis = 1; axis <= 3; ++axis) {
-            const uint32_t s = kAxis1 + axis - 1;
-            hipLaunchKernelGGL(prefix_axis_batch_kernel, dim3(grid[s]), dim3(256), 0, stream, d_inst, st + s * (N + 1),
-                               N, axis);
+        if (fused_prefix) {
+            hipLaunchKernelGGL(prefix_plane_batch_kernel, dim3(grid[kPlane]), dim3(kPlaneThreads), plane_lds, stream,
+                               d_inst, st + kPlane * (N + 1), N);
+            hipLaunchKernelGGL(prefix_axis_batch_kernel, dim3(grid[kAxis2]), dim3(256), 0, stream, d_inst,
+                               st + kAxis2 * (N + 1), N, 2);
+            hipLaunchKernelGGL(prefix_last_axis_batch_kernel, dim3(grid[kLast]), dim3(256), 0, stream, d_inst,
+                               st + kLast * (N + 1), N);
+        } else {
+            hipLaunchKernelGGL(prefix_rows_batch_kernel, dim3(grid[kRows]), dim3(256), 0, stream, d_inst,
+                               st + kRows * (N + 1), N);
+            for (int axis = 1; axis <= 3; ++axis) {
+                const uint32_t s = kAxis1 + axis - 1;
+                hipLaunchKernelGGL(prefix_axis_batch_kernel, dim3(grid[s]), dim3(256), 0, stream, d_inst, st + s * (N + 1),
+                                   N, axis);
+            }
+            hipLaunchKernelGGL(coarse_batch_kernel, dim3(grid[kCoarseStage]), dim3(256), 0, stream, d_inst,
+                               st + kCoarseStage * (N + 1), N);
+            hipLaunchKernelGGL(unreached_bits_batch_kernel, dim3(grid[kBits]), dim3(256), 0, stream, d_inst,
+                               st + kBits * (N + 1), N);
         }
-        hipLaunchKernelGGL(coarse_batch_kernel, dim3(grid[kCoarseStage]), dim3(256), 0, stream, d_inst,
-                           st + kCoarseStage * (N + 1), N);
-        hipLaunchKernelGGL(unreached_bits_batch_kernel, dim3(grid[kBits]), dim3(256), 0, stream, d_inst,
-                           st + kBits * (N + 1), N);
         hipLaunchKernelGGL(chunk_flags_batch_kernel, dim3(grid[kChunk]), dim3(256), 0, stream, d_inst,
                            st + kChunk * (N + 1), N);
         GSC_HIP(launch_satisfy_batch(d_inst, st + kSat * (N + 1), N, grid[kSat], static_cast<uint32_t>(opts.tile_wg),

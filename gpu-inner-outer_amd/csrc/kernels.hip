@@ -321,22 +321,13 @@ __global__ __launch_bounds__(256) void prefix_axis_kernel(DevCase c, int axis, u
 // LDS layout: element (x, y) at y * pitch + (x ^ (y & 63)), pitch a multiple
 // of 64, so that rows (lanes along x) and columns (lanes along y) both hit 64
 // different banks.
-__global__ __launch_bounds__(kPlaneThreads) void prefix_plane_kernel(DevCase c, const uint8_t* __restrict__ a,
-                                                                      const uint8_t* __restrict__ b, uint32_t* P0,
-                                                                      uint32_t* P1,
-                                                                      unsigned long long* __restrict__ bits,
-                                                                      uint32_t pitch, const IterCtl* ctl) {
+__device__ __forceinline__ void prefix_plane_body(const DevCase& c, const uint8_t* __restrict__ a,
+                                                  const uint8_t* __restrict__ b, uint32_t* __restrict__ P,
+                                                  unsigned long long* __restrict__ bits, uint32_t pitch, uint32_t plane) {
     extern __shared__ uint32_t sm[];
-    uint32_t* P = P0;
-    if (ctl) {
-        const uint64_t st = loop_state(ctl);
-        if (state_done(st)) return;
-        P = (state_next_iteration(st) & 1u) ? P1 : P0;
-    }
     constexpr uint32_t kW = kPlaneThreads / 64;
     const uint32_t lane = lane_id();
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t plane = blockIdx.x;
     const uint32_t n0 = c.shape[0], n1 = c.shape[1], PQ = c.PQ;
     const uint32_t base = plane * PQ;
     if (bits) {
@@ -393,22 +384,26 @@ __global__ __launch_bounds__(kPlaneThreads) void prefix_plane_kernel(DevCase c, 
     }
 }
 
+__global__ __launch_bounds__(kPlaneThreads) void prefix_plane_kernel(DevCase c, const uint8_t* __restrict__ a,
+                                                                      const uint8_t* __restrict__ b, uint32_t* P0,
+                                                                      uint32_t* P1,
+                                                                      unsigned long long* __restrict__ bits,
+                                                                      uint32_t pitch, const IterCtl* ctl) {
+    uint32_t* P = P0;
+    if (ctl) {
+        const uint64_t st = loop_state(ctl);
+        if (state_done(st)) return;
+        P = (state_next_iteration(st) & 1u) ? P1 : P0;
+    }
+    prefix_plane_body(c, a, b, P, bits, pitch, blockIdx.x);
+}
+
 // The last (v) axis pass with the coarse sample fused: the thread of line
 // (x, y, θ) writes C[X, Y, θ + 1, v + 1] for every v when x + 1 and y + 1 are
 // coarse sample positions (multiples of kCoarse, or the last index); the
 // coarse entries on the padding planes stay zero from set-up.
-__global__ __launch_bounds__(256) void prefix_last_axis_kernel(DevCase c, uint32_t* P0, uint32_t* P1, uint32_t* C0,
-                                                                uint32_t* C1, const IterCtl* ctl) {
-    uint32_t* P = P0;
-    uint32_t* C = C0;
-    if (ctl) {
-        const uint64_t st = loop_state(ctl);
-        if (state_done(st)) return;
-        const bool odd = (state_next_iteration(st) & 1u) != 0;
-        P = odd ? P1 : P0;
-        C = odd ? C1 : C0;
-    }
-    const uint32_t tid = blockIdx.x * 256u + threadIdx.x;
+__device__ __forceinline__ void prefix_last_axis_body(const DevCase& c, uint32_t* __restrict__ P,
+                                                      uint32_t* __restrict__ C, uint32_t tid) {
     const uint32_t n0 = c.shape[0], n1 = c.shape[1];
     const uint32_t lines = n0 * n1 * c.shape[2];
     if (tid >= lines) return;
@@ -441,6 +436,20 @@ __global__ __launch_bounds__(256) void prefix_last_axis_kernel(DevCase c, uint32
         off += kLineBatch * step;
         coff += kLineBatch * c.cstride[3];
     }
+}
+
+__global__ __launch_bounds__(256) void prefix_last_axis_kernel(DevCase c, uint32_t* P0, uint32_t* P1, uint32_t* C0,
+                                                                uint32_t* C1, const IterCtl* ctl) {
+    uint32_t* P = P0;
+    uint32_t* C = C0;
+    if (ctl) {
+        const uint64_t st = loop_state(ctl);
+        if (state_done(st)) return;
+        const bool odd = (state_next_iteration(st) & 1u) != 0;
+        P = odd ? P1 : P0;
+        C = odd ? C1 : C0;
+    }
+    prefix_last_axis_body(c, P, C, blockIdx.x * 256u + threadIdx.x);
 }
 
 // Coarse sample: C[X, Y, z, w] = P[min(kCoarse X, n_x), min(kCoarse Y, n_y), z, w]
@@ -1343,6 +1352,31 @@ __global__ __launch_bounds__(256) void coarse_batch_kernel(const BatchInstance* 
     const uint32_t par = state_next_iteration(st) & 1u;
     coarse_body(I.sat.c, I.prefix_unreached[par], I.coarse_unreached[par],
                 (blockIdx.x - starts[k]) * 256u + threadIdx.x);
+}
+
+// The fused prefix build of every instance whose planes fit in LDS (the
+// batch uses it only when all do): a workgroup per plane, the case's pitch in
+// BatchInstance::plane_pitch, dynamic LDS sized for the largest plane.
+__global__ __launch_bounds__(kPlaneThreads) void prefix_plane_batch_kernel(const BatchInstance* __restrict__ B,
+                                                                            const uint32_t* __restrict__ starts,
+                                                                            uint32_t n) {
+    const uint32_t k = find_instance(starts, n, blockIdx.x);
+    const BatchInstance& I = B[k];
+    const uint64_t st = loop_state(I.sat.ctl);
+    if (state_done(st)) return;
+    prefix_plane_body(I.sat.c, I.sat.valid, I.sat.reach, I.prefix_unreached[state_next_iteration(st) & 1u],
+                      I.unreached_bits, I.plane_pitch, blockIdx.x - starts[k]);
+}
+
+__global__ __launch_bounds__(256) void prefix_last_axis_batch_kernel(const BatchInstance* __restrict__ B,
+                                                                      const uint32_t* __restrict__ starts, uint32_t n) {
+    const uint32_t k = find_instance(starts, n, blockIdx.x);
+    const BatchInstance& I = B[k];
+    const uint64_t st = loop_state(I.sat.ctl);
+    if (state_done(st)) return;
+    const uint32_t par = state_next_iteration(st) & 1u;
+    prefix_last_axis_body(I.sat.c, I.prefix_unreached[par], I.coarse_unreached[par],
+                          (blockIdx.x - starts[k]) * 256u + threadIdx.x);
 }
 
 // Chunk flags of every instance (chunk_flags_kernel), into each case's own table.

@@ -122,7 +122,7 @@ struct BatchInstance {
     IterRecord* records;
     unsigned long long candidate_count;
     uint32_t max_iterations;
-    uint32_t pad;
+    uint32_t plane_pitch;  // LDS row pitch of the fused prefix build (0: the plane does not fit)
 };
 
 // Per-instance state published by finish_batch_kernel after every iteration.
@@ -184,6 +184,8 @@ __global__ void unreached_bits_batch_kernel(const BatchInstance* B, const uint32
 __global__ void finish_batch_kernel(const BatchInstance* B, uint32_t n, BatchStatus* status);
 __global__ void seed_batch_kernel(const BatchInstance* B, const uint32_t* starts, uint32_t n);
 __global__ void chunk_flags_batch_kernel(const BatchInstance* B, const uint32_t* starts, uint32_t n);
+__global__ void prefix_plane_batch_kernel(const BatchInstance* B, const uint32_t* starts, uint32_t n);
+__global__ void prefix_last_axis_batch_kernel(const BatchInstance* B, const uint32_t* starts, uint32_t n);
 hipError_t launch_satisfy_batch(const BatchInstance* B, const uint32_t* starts, uint32_t n, uint32_t blocks,
                                 uint32_t wg, uint32_t ib, hipStream_t stream);
 
