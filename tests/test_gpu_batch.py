@@ -86,3 +86,28 @@ def test_batch_config_error_names_case(gpu):
     with pytest.raises(GscError) as ei:
         Batch([good, bad])
     assert "case 1" in str(ei.value) and "candidate set contains invalid states" in str(ei.value)
+
+
+def _wide_plane_case():
+    # 100 x 129 cells per (θ, v) plane: 128-cell rows x 129 rows of 4 bytes
+    # exceed the 64 KiB of LDS the fused prefix build uses.
+    lb, ub, eta = cases._domain((100, 129, 24, 3))
+    ilb, iub, ieta = cases._inputs(3)
+    return cases.Case(
+        name="plane_100x129", state_lb=lb, state_ub=ub, state_eta=eta, input_lb=ilb, input_ub=iub, input_eta=ieta,
+        candidate=cases._r((4.0, 4.0, -cases.PI_LITERAL, 0.0), (6.0, 6.0, cases.PI_LITERAL, 2.0)),
+        map=cases._r(lb, ub), obstacles=[cases._r((1.0, 7.0, lb[2], lb[3]), (3.0, 8.0, ub[2], ub[3]))],
+        sample_time=0.5, integration_substeps=32, max_iterations=25, disturbance_half_width=(0.0, 0.0, 0.0, 0.0),
+    )
+
+
+def test_batch_with_a_plane_too_large_for_lds(gpu, oracle):
+    # One case whose plane does not fit in LDS sends the whole batch through the
+    # separate prefix passes; every case must still match the oracle.
+    cs = [cases.unicycle_30x30x24x10(sample_time=0.5), _wide_plane_case(), cases.builtin("default")]
+    print(f"[case] batch with a wide plane: {[c.name for c in cs]}", flush=True)
+    refs = [oracle.run(c, keep_abstraction=False) for c in cs]
+    with Batch(cs) as b:
+        b.build_abstraction()
+        b.solve()
+        _check_all(b, cs, refs, "wide plane")
