@@ -324,14 +324,14 @@ GSC_HD inline bool cell_offsets(const DevCase& c, int d, double qa, double qb, C
 }
 
 // The cells of cell i from its (plane, input) offsets: false when the box
-// leaves the domain (dim_to_cells' rejection), else the clamped range.
+// leaves the domain (dim_to_cells' rejection). The clamped range is written
+// either way (cells in [0, n - 1]), so callers select rather than branch.
 GSC_HD inline bool cells_from_offsets(const CellOffsets& o, int32_t i, uint32_t n, uint32_t* lo, uint32_t* hi) {
-    if (i < o.rej_lo || i > o.rej_hi) return false;
     const int32_t top = static_cast<int32_t>(n) - 1;
     const int32_t a = i + o.off_lo, b = i + o.off_hi;
     *lo = static_cast<uint32_t>(a < 0 ? 0 : a > top ? top : a);
     *hi = static_cast<uint32_t>(b < 0 ? 0 : b > top ? top : b);
-    return true;
+    return (i >= o.rej_lo) & (i <= o.rej_hi);
 }
 
 // 16-corner inclusion–exclusion on a padded u32 prefix table (prefix_sum.hpp:65-85)

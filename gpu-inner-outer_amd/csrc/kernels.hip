@@ -645,44 +645,53 @@ __device__ __forceinline__ bool pair_box(const DevCase& c, const DevCase* cg, co
                                          const int32_t offs[8], uint2 b2, uint2 b3, uint32_t ix, uint32_t iy, double x0,
                                          double y0, double rad0, double rad1, const uint32_t* __restrict__ fail2d,
                                          uint32_t lo[4], uint32_t hi[4]) {
-    if (b2.x == kBoxReject || b3.x == kBoxReject) return false;
     lo[2] = b2.x;
     hi[2] = b2.y;
     lo[3] = b3.x;
     hi[3] = b3.y;
-    // Fast path (x and y are plain intervals when the wrap axis is θ or v): the
-    // (plane, input)'s cell offsets, integer arithmetic only, when they decided
-    // (a wave-uniform test); else filtered_cells.
+    if (b2.x == kBoxReject || b3.x == kBoxReject) {
+        lo[0] = hi[0] = lo[1] = hi[1] = 0;
+        return false;
+    }
+    // The branches below are wave-uniform (`inc` and the offsets belong to one
+    // (plane, input)); the per-lane tests are selects, so a wave spends no
+    // exec-mask bookkeeping per pair. Fast path (x and y are plain intervals
+    // when the wrap axis is θ or v): the (plane, input)'s cell offsets, integer
+    // arithmetic only, when they decided; else filtered_cells.
+    bool ok;
     if (c.wrap_dim >= 2 && offs[0] != kOffsetsUndecided && offs[4] != kOffsetsUndecided) {
         const CellOffsets ox{offs[0], offs[1], offs[2], offs[3]};
         const CellOffsets oy{offs[4], offs[5], offs[6], offs[7]};
-        if (!cells_from_offsets(ox, static_cast<int32_t>(ix), c.shape[0], &lo[0], &hi[0]) ||
-            !cells_from_offsets(oy, static_cast<int32_t>(iy), c.shape[1], &lo[1], &hi[1]))
-            return false;
+        ok = cells_from_offsets(ox, static_cast<int32_t>(ix), c.shape[0], &lo[0], &hi[0]) &
+             cells_from_offsets(oy, static_cast<int32_t>(iy), c.shape[1], &lo[1], &hi[1]);
     } else {
         const ExactCells e = filtered_cells(cg, inc, sums, x0, y0, rad0, rad1);
-        if (!e.ok) return false;
-        lo[0] = e.lo0;
-        hi[0] = e.hi0;
-        lo[1] = e.lo1;
-        hi[1] = e.hi1;
+        ok = e.ok != 0;
+        // A rejected pair's cells are unspecified: zero them, so the table
+        // reads below stay in bounds.
+        lo[0] = ok ? e.lo0 : 0u;
+        hi[0] = ok ? e.hi0 : 0u;
+        lo[1] = ok ? e.lo1 : 0u;
+        hi[1] = ok ? e.hi1 : 0u;
     }
     if (c.constraint_type != GSC_CONSTRAINT_NONE) {
         // Every loop of abstraction.hpp:124-156 must be non-empty for a cell to be
         // tested; the test reads x and y only, so it reduces to "no failing (x, y)
         // cell in the x/y rectangle". The θ walk is empty only when its range is
-        // inverted without a wrap (abstraction.hpp:125-135).
+        // inverted without a wrap (abstraction.hpp:125-135). Cells lie in
+        // [0, shape - 1], so the four reads are in bounds on every lane.
+        // Bitwise, not short-circuit, so the θ / v terms stay scalar.
         const int wd = c.wrap_dim;
-        const bool has_wrap = lo[wd] > hi[wd];
-        const bool empty = lo[3] > hi[3] || (lo[2] > hi[2] && !has_wrap) || lo[1] > hi[1] || lo[0] > hi[0];
-        if (!empty) {
-            const uint32_t fpitch = c.shape[0] + 1;
-            const uint32_t f = fail2d[(hi[1] + 1) * fpitch + hi[0] + 1] - fail2d[(hi[1] + 1) * fpitch + lo[0]] -
-                               fail2d[lo[1] * fpitch + hi[0] + 1] + fail2d[lo[1] * fpitch + lo[0]];
-            if (f != 0) return false;
-        }
+        const bool inv2 = b2.x > b2.y, inv3 = b3.x > b3.y;
+        const bool has_wrap = ((wd == 2) & inv2) | ((wd == 3) & inv3) | ((wd == 1) & (lo[1] > hi[1])) |
+                              ((wd == 0) & (lo[0] > hi[0]));
+        const bool empty = inv3 | (inv2 & !has_wrap) | (lo[1] > hi[1]) | (lo[0] > hi[0]);
+        const uint32_t fpitch = c.shape[0] + 1;
+        const uint32_t f = fail2d[(hi[1] + 1) * fpitch + hi[0] + 1] - fail2d[(hi[1] + 1) * fpitch + lo[0]] -
+                           fail2d[lo[1] * fpitch + hi[0] + 1] + fail2d[lo[1] * fpitch + lo[0]];
+        ok = ok & (empty | (f == 0));
     }
-    return true;
+    return ok;
 }
 
 // The state's own centre must satisfy the constraint (abstraction.hpp:177-181).
@@ -755,6 +764,9 @@ __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
     // lane, padding inputs (I to I_pad) included as kRecInvalid.
     constexpr uint32_t kAbsGroup = 8;
     static_assert(kRecBlock == kAbsGroup, "a group is one record block");
+    const bool wave_states_ok = __ballot(state_ok) != 0;
+    // (iw, ia) of input i, advanced without a division.
+    uint32_t iw_next = 0, ia_next = 0;
     for (uint32_t i0 = 0; i0 < c.I; i0 += kAbsGroup) {
         uint32_t rr[kAbsGroup];
 #pragma unroll
@@ -762,10 +774,16 @@ __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
             const uint32_t i = i0 + j;
             rr[j] = kRecInvalid;
             if (i >= c.I) continue;
-            const uint32_t iw = i / c.n_a, ia = i - iw * c.n_a;
+            const uint32_t iw = iw_next, ia = ia_next;
+            if (++ia_next == c.n_a) {
+                ia_next = 0;
+                ++iw_next;
+            }
+            // A wave whose states all fail the constraint (or lie past the
+            // plane) has no valid pair: its records stay invalid.
+            if (!wave_states_ok) continue;
             const uint2 b2 = make_uint2(box2[2 * iw], box2[2 * iw + 1]);
             const uint2 b3 = make_uint2(box3[2 * ia], box3[2 * ia + 1]);
-            uint32_t r = kRecInvalid;
             uint32_t lo[4], hi[4];
             const double2* const inc = A.inc + (static_cast<size_t>(plane) * c.I + i) * c.K;
             const double* const sums = A.inc_sums + (static_cast<size_t>(plane) * c.I + i) * 4;
@@ -774,34 +792,29 @@ __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
             int32_t offs[8];
 #pragma unroll
             for (int k = 0; k < 8; ++k) offs[k] = offs_p[k];
-            const bool ok = state_ok && pair_box(c, &kernarg_block<AbstractionArgs>()->c, inc, sums, offs, b2, b3, ix,
-                                                 iy, x0, y0, rad0, rad1, A.fail2d, lo, hi);
-            if (ok) {
-                // Inverted ranges on a non-wrap axis or wrapped x/y ranges take the escape path.
-                const bool regular = lo[0] <= hi[0] && lo[1] <= hi[1] && (lo[2] <= hi[2] || wd == 2) &&
-                                     (lo[3] <= hi[3] || wd == 3);
-                const int off0 = static_cast<int>(lo[0]) - static_cast<int>(ix);
-                const int off1 = static_cast<int>(lo[1]) - static_cast<int>(iy);
-                const bool fits = regular && off0 >= kOffMin && off0 <= kOffMax && off1 >= kOffMin &&
-                                  off1 <= kOffMax && hi[0] - lo[0] <= kExtMax && hi[1] - lo[1] <= kExtMax;
-                any_valid = true;
-                if (fits) {
-                    hx0 = min(hx0, off0);
-                    hx1 = max(hx1, off0 + static_cast<int>(hi[0] - lo[0]));
-                    hy0 = min(hy0, off1);
-                    hy1 = max(hy1, off1 + static_cast<int>(hi[1] - lo[1]));
-                } else {
-                    unrepresentable = true;
-                }
-                if (!store) {
-                    // hulls only
-                } else if (fits) {
-                    r = rec_pack(off0, hi[0] - lo[0], off1, hi[1] - lo[1], false);
-                } else {
-                    r = escape_record(kernarg_block<AbstractionArgs>(), lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2],
-                                      hi[3]);
-                }
-            }
+            const bool ok = pair_box(c, &kernarg_block<AbstractionArgs>()->c, inc, sums, offs, b2, b3, ix, iy, x0, y0,
+                                     rad0, rad1, A.fail2d, lo, hi) &
+                            state_ok;
+            // Per-lane outcome as selects: inverted ranges on a non-wrap axis or
+            // wrapped x/y ranges take the escape path.
+            const bool regular = (lo[0] <= hi[0]) & (lo[1] <= hi[1]) & ((b2.x <= b2.y) | (wd == 2)) &
+                                 ((b3.x <= b3.y) | (wd == 3));
+            const int off0 = static_cast<int>(lo[0]) - static_cast<int>(ix);
+            const int off1 = static_cast<int>(lo[1]) - static_cast<int>(iy);
+            const uint32_t ext0 = hi[0] - lo[0], ext1 = hi[1] - lo[1];
+            const bool fits = regular & (off0 >= kOffMin) & (off0 <= kOffMax) & (off1 >= kOffMin) &
+                              (off1 <= kOffMax) & (ext0 <= kExtMax) & (ext1 <= kExtMax);
+            const bool packed = ok & fits;
+            any_valid |= ok;
+            unrepresentable |= ok & !fits;
+            hx0 = packed ? min(hx0, off0) : hx0;
+            hx1 = packed ? max(hx1, off0 + static_cast<int>(ext0)) : hx1;
+            hy0 = packed ? min(hy0, off1) : hy0;
+            hy1 = packed ? max(hy1, off1 + static_cast<int>(ext1)) : hy1;
+            uint32_t r = packed ? rec_pack(off0, ext0, off1, ext1, false) : kRecInvalid;
+            if (store && ok && !fits)
+                r = escape_record(kernarg_block<AbstractionArgs>(), lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2],
+                                  hi[3]);
             rr[j] = r;
         }
         if (live && store) {
