@@ -139,6 +139,9 @@ struct DevCase {
     int32_t band_enabled;
     double band_c, band_w;
     uint32_t n_obstacles, n_targets;
+    // Row pitch of the 2D constraint-failure prefix table: 1 << fail_shift
+    // entries, the least power of two >= n_x + 1, so a row offset is a shift.
+    uint32_t fail_shift;
     int32_t hull_enabled;  // frontier skipping available (wrap along θ or v only)
     // Coarse prefix tables: the full tables sampled at x, y multiples of
     // kCoarse (and at n_x, n_y), θ and v at full resolution.

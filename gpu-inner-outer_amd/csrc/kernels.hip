@@ -686,9 +686,13 @@ __device__ __forceinline__ bool pair_box(const DevCase& c, const DevCase* cg, co
         const bool has_wrap = ((wd == 2) & inv2) | ((wd == 3) & inv3) | ((wd == 1) & (lo[1] > hi[1])) |
                               ((wd == 0) & (lo[0] > hi[0]));
         const bool empty = inv3 | (inv2 & !has_wrap) | (lo[1] > hi[1]) | (lo[0] > hi[0]);
-        const uint32_t fpitch = c.shape[0] + 1;
-        const uint32_t f = fail2d[(hi[1] + 1) * fpitch + hi[0] + 1] - fail2d[(hi[1] + 1) * fpitch + lo[0]] -
-                           fail2d[lo[1] * fpitch + hi[0] + 1] + fail2d[lo[1] * fpitch + lo[0]];
+        // 32-bit byte offsets (the table is under 4 GiB) against the table's base.
+        const uint32_t sh = c.fail_shift + 2;
+        const uint32_t r0 = lo[1] << sh, r1 = (hi[1] + 1) << sh;
+        const uint32_t c0 = lo[0] << 2, c1 = (hi[0] + 1) << 2;
+        const char* const fb = reinterpret_cast<const char*>(fail2d);
+        const auto at = [fb](uint32_t off) { return *reinterpret_cast<const uint32_t*>(fb + off); };
+        const uint32_t f = at(r1 + c1) - at(r1 + c0) - at(r0 + c1) + at(r0 + c0);
         ok = ok & (empty | (f == 0));
     }
     return ok;
@@ -698,9 +702,8 @@ __device__ __forceinline__ bool pair_box(const DevCase& c, const DevCase* cg, co
 __device__ __forceinline__ bool state_constraint_ok(const DevCase& c, const uint32_t* __restrict__ fail2d, uint32_t ix,
                                                     uint32_t iy) {
     if (c.constraint_type == GSC_CONSTRAINT_NONE) return true;
-    const uint32_t fpitch = c.shape[0] + 1;
-    const uint32_t f = fail2d[(iy + 1) * fpitch + ix + 1] - fail2d[(iy + 1) * fpitch + ix] -
-                       fail2d[iy * fpitch + ix + 1] + fail2d[iy * fpitch + ix];
+    const uint32_t r0 = iy << c.fail_shift, r1 = (iy + 1) << c.fail_shift;
+    const uint32_t f = fail2d[r1 + ix + 1] - fail2d[r1 + ix] - fail2d[r0 + ix + 1] + fail2d[r0 + ix];
     return f == 0;
 }
 
@@ -765,8 +768,11 @@ __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
     constexpr uint32_t kAbsGroup = 8;
     static_assert(kRecBlock == kAbsGroup, "a group is one record block");
     const bool wave_states_ok = __ballot(state_ok) != 0;
-    // (iw, ia) of input i, advanced without a division.
-    uint32_t iw_next = 0, ia_next = 0;
+    // Per-input table rows, advanced input by input (i = iw * n_a + ia)
+    // without a division or a product.
+    ConstPtr<uint32_t> b2_row = box2, b3_row = box3;
+    ConstPtr<int32_t> offs_row = to_const(A.cell_offs + static_cast<size_t>(plane) * c.I * 8);
+    uint32_t ia_next = 0;
     for (uint32_t i0 = 0; i0 < c.I; i0 += kAbsGroup) {
         uint32_t rr[kAbsGroup];
 #pragma unroll
@@ -774,21 +780,24 @@ __global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
             const uint32_t i = i0 + j;
             rr[j] = kRecInvalid;
             if (i >= c.I) continue;
-            const uint32_t iw = iw_next, ia = ia_next;
+            const ConstPtr<uint32_t> b2p = b2_row, b3p = b3_row;
+            const ConstPtr<int32_t> offs_p = offs_row;
+            offs_row += 8;
+            b3_row += 2;
             if (++ia_next == c.n_a) {
                 ia_next = 0;
-                ++iw_next;
+                b3_row = box3;
+                b2_row += 2;
             }
             // A wave whose states all fail the constraint (or lie past the
             // plane) has no valid pair: its records stay invalid.
             if (!wave_states_ok) continue;
-            const uint2 b2 = make_uint2(box2[2 * iw], box2[2 * iw + 1]);
-            const uint2 b3 = make_uint2(box3[2 * ia], box3[2 * ia + 1]);
+            const uint2 b2 = make_uint2(b2p[0], b2p[1]);
+            const uint2 b3 = make_uint2(b3p[0], b3p[1]);
             uint32_t lo[4], hi[4];
             const double2* const inc = A.inc + (static_cast<size_t>(plane) * c.I + i) * c.K;
             const double* const sums = A.inc_sums + (static_cast<size_t>(plane) * c.I + i) * 4;
             // The (plane, input)'s cell offsets in one scalar load, ahead of the branches that use them.
-            const ConstPtr<int32_t> offs_p = to_const(A.cell_offs + (static_cast<size_t>(plane) * c.I + i) * 8);
             int32_t offs[8];
 #pragma unroll
             for (int k = 0; k < 8; ++k) offs[k] = offs_p[k];

@@ -60,7 +60,7 @@ struct AbstractionArgs {
     const uint2* box2;           // [n_θ][n_ω] θ cell range (or kBoxReject)
     const uint2* box3;           // [n_v][n_a] v cell range (or kBoxReject)
     const double* rad;           // [n_v][2] x / y box radius
-    const uint32_t* fail2d;      // (n0+1)x(n1+1) prefix of constraint failures, or null
+    const uint32_t* fail2d;      // 2D prefix of constraint failures ((n1+1) rows of 1 << fail_shift), or null
     const uint32_t* prefix_valid;
     uint32_t* rec;                // null: compute the state hulls only (recompute mode)
     OverflowEntry* overflow;

@@ -119,8 +119,8 @@ void gsc_solver::build_tables(std::vector<double2>& h_cs, std::vector<double>& h
     h_fail.clear();
     if (cfg.constraint_type != GSC_CONSTRAINT_NONE) {
         const uint32_t n0 = dc.shape[0], n1 = dc.shape[1];
-        const uint32_t pitch = n0 + 1;
-        h_fail.assign(static_cast<size_t>(n0 + 1) * (n1 + 1), 0u);
+        const size_t pitch = size_t{1} << dc.fail_shift;
+        h_fail.assign((n1 + 1) * pitch, 0u);
         for (uint32_t iy = 0; iy < n1; ++iy) {
             const double yc = cell_center(cfg.state_lb[1], iy, cfg.state_eta[1]);
             uint32_t row = 0;
@@ -173,6 +173,10 @@ void gsc_solver::setup() {
     dc.n_a = shape.input_shape[0];
     dc.n_w = shape.input_shape[1];
     dc.K = cfg.integration_substeps;
+    dc.fail_shift = 0;
+    while ((1u << dc.fail_shift) < dc.shape[0] + 1) ++dc.fail_shift;
+    if ((static_cast<uint64_t>(dc.shape[1]) + 1) << (dc.fail_shift + 2) > 0xFFFFFFFFull)
+        throw ConfigError("x/y grid too large for the 32-bit constraint table offsets");
     dc.dt = cfg.sample_time / static_cast<double>(cfg.integration_substeps);
     dc.constraint_type = cfg.constraint_type;
     dc.hyp_a = cfg.hyperbolic_a;
