@@ -733,7 +733,7 @@ __device__ __forceinline__ uint32_t escape_record(const AbstractionArgs* __restr
 // Abstraction: every pair's box, packed into its record (see gsc_device.hpp).
 // The `dead` bit of compact records is left 0 here and settled lazily by the
 // satisfaction pass (see satisfy_kernel); escaped pairs get it eagerly.
-__global__ __launch_bounds__(256) void abstraction_kernel(AbstractionArgs A) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void abstraction_kernel(AbstractionArgs A) {
     const DevCase& c = A.c;
     const uint32_t lane = lane_id();
     const uint32_t task = blockIdx.x * 4u + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
