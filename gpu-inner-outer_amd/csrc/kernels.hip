@@ -213,7 +213,7 @@ __device__ __forceinline__ void prefix_rows_body(const DevCase& c, const uint8_t
             // Both masks loaded unconditionally: a short-circuit would load b only
             // after a had arrived, two memory round trips instead of one.
             const uint32_t av = a[src + x], bv = b ? b[src + x] : 0u;
-            v = (av != 0) & (bv == 0) ? 1u : 0u;
+            v = ((av != 0) & (bv == 0)) ? 1u : 0u;
         }
         v = wave_inclusive_scan(v) + carry;
         if (x < c.shape[0]) P[dst + x] = v;
@@ -356,7 +356,7 @@ __device__ __forceinline__ void prefix_plane_body(const DevCase& c, const uint8_
                 av = a[q];  // both loads issued together (no short-circuit)
                 bv = b ? b[q] : 0u;
             }
-            uint32_t v = (av != 0) & (bv == 0) ? 1u : 0u;
+            uint32_t v = ((av != 0) & (bv == 0)) ? 1u : 0u;
             v = wave_inclusive_scan(v) + carry;
             if (x < n0) sm[y * pitch + (x ^ sw)] = v;
             carry = last_lane(v);
@@ -662,8 +662,9 @@ __device__ __forceinline__ bool pair_box(const DevCase& c, const DevCase* cg, co
     if (c.wrap_dim >= 2 && offs[0] != kOffsetsUndecided && offs[4] != kOffsetsUndecided) {
         const CellOffsets ox{offs[0], offs[1], offs[2], offs[3]};
         const CellOffsets oy{offs[4], offs[5], offs[6], offs[7]};
-        ok = cells_from_offsets(ox, static_cast<int32_t>(ix), c.shape[0], &lo[0], &hi[0]) &
-             cells_from_offsets(oy, static_cast<int32_t>(iy), c.shape[1], &lo[1], &hi[1]);
+        const bool ok_x = cells_from_offsets(ox, static_cast<int32_t>(ix), c.shape[0], &lo[0], &hi[0]);
+        const bool ok_y = cells_from_offsets(oy, static_cast<int32_t>(iy), c.shape[1], &lo[1], &hi[1]);
+        ok = ok_x & ok_y;
     } else {
         const ExactCells e = filtered_cells(cg, inc, sums, x0, y0, rad0, rad1);
         ok = e.ok != 0;
