@@ -1158,7 +1158,7 @@ __global__ __launch_bounds__(256) void chunk_flags_kernel(SatisfyArgs A, uint8_t
 // stored records: no per-pair HBM at all, at the cost of the K trajectory steps
 // per tested pair. Same schedule and results as satisfy_kernel; one input at a
 // time per lane, lanes leave at their first satisfied input.
-__global__ __launch_bounds__(256) void satisfy_recompute_kernel(RecomputeArgs R) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void satisfy_recompute_kernel(RecomputeArgs R) {
     const SatisfyArgs& A = R.sat;
     const DevCase& c = A.c;
     constexpr uint32_t kUse = 1, kCand = 2, kReached = 4, kCertified = 8;
