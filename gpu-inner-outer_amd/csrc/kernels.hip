@@ -1424,7 +1424,7 @@ __device__ __forceinline__ void satisfy_in_batch(const SatisfyArgs& A, uint32_t 
 }
 
 template <uint32_t WG, uint32_t IB>
-__global__ __launch_bounds__(WG) void satisfy_batch_kernel(const BatchInstance* __restrict__ B,
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) void satisfy_batch_kernel(const BatchInstance* __restrict__ B,
                                                            const uint32_t* __restrict__ starts, uint32_t n) {
     const uint32_t inst = find_instance(starts, n, blockIdx.x);
     const SatisfyArgs& A = B[inst].sat;
