@@ -105,8 +105,10 @@ __device__ inline void unflatten(const DevCase& c, uint32_t s, uint32_t idx[4]) 
 
 // ---------------------------------------------------------------------------
 // Masks and seeds. Thread per state.
+// `allow` (may be null) removes states a caller-side pointwise constraint rejects.
 __global__ __launch_bounds__(256) void masks_kernel(DevCase c, const gsc_rect4* __restrict__ obstacles,
                                                      const gsc_rect4* __restrict__ targets,
+                                                     const uint8_t* __restrict__ allow,
                                                      uint8_t* __restrict__ valid, uint8_t* __restrict__ cand,
                                                      unsigned long long* __restrict__ counts) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -125,6 +127,7 @@ __global__ __launch_bounds__(256) void masks_kernel(DevCase c, const gsc_rect4* 
                 }
         if (ok && c.constraint_type != GSC_CONSTRAINT_NONE)
             ok = constraint_ok(c.constraint_type, c.hyp_a, c.hyp_b, c.hyp_c, c.ell_a, c.ell_b, x[0], x[1]);
+        if (ok && allow) ok = allow[s] != 0;
         is_cand = point_in_rect(c.candidate, x);
         if (is_cand && c.band_enabled) is_cand = fabs(x[0] * x[1] - c.band_c) <= c.band_w;
         if (!is_cand)

@@ -133,8 +133,8 @@ struct BatchStatus {
     unsigned long long total_certified;
 };
 
-__global__ void masks_kernel(DevCase c, const gsc_rect4* obstacles, const gsc_rect4* targets, uint8_t* valid,
-                             uint8_t* cand, unsigned long long* counts);
+__global__ void masks_kernel(DevCase c, const gsc_rect4* obstacles, const gsc_rect4* targets, const uint8_t* allow,
+                             uint8_t* valid, uint8_t* cand, unsigned long long* counts);
 __global__ void seed_kernel(uint32_t S, const uint8_t* cand, uint8_t* reach, uint32_t* ctrl, uint32_t* rstep);
 __global__ void reset_ctl_kernel(IterCtl* ctl, unsigned long long candidate_count);
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t value);

@@ -550,7 +550,7 @@ int gsc_server_submit(gsc_server* s, const gsc_case_desc* cfg, uint64_t* id) {
     try {
         r->shape = make_shape(r->spec->cfg);
         if (r->spec->cfg.wrap_dim < 0 || r->spec->cfg.wrap_dim > 3) throw ConfigError("wrap_dim must be 0, 1, 2 or 3");
-        r->est_bytes = gsc_solver::device_bytes(r->shape, r->spec->cfg.integration_substeps);
+        r->est_bytes = gsc_solver::device_bytes(r->shape, r->spec->cfg);
     } catch (const std::exception& e) {
         rc = GSC_ERR_CONFIG;
         why = e.what();

@@ -279,6 +279,7 @@ void gsc_solver::setup() {
     cell_offs = alloc<int32_t>(8ull * dc.shape[2] * dc.shape[3] * dc.I);
     d_obstacles = alloc<gsc_rect4>(obstacles.size());
     d_targets = alloc<gsc_rect4>(targets.size());
+    uint8_t* d_allow = allow.empty() ? nullptr : alloc<uint8_t>(S);
 
     const auto t0 = std::chrono::steady_clock::now();
     GSC_HIP(hipMemcpyAsync(cs, h_cs.data(), h_cs.size() * sizeof(double2), hipMemcpyHostToDevice, stream));
@@ -296,6 +297,7 @@ void gsc_solver::setup() {
     if (!targets.empty())
         GSC_HIP(hipMemcpyAsync(d_targets, targets.data(), targets.size() * sizeof(gsc_rect4), hipMemcpyHostToDevice,
                                stream));
+    if (d_allow) GSC_HIP(hipMemcpyAsync(d_allow, allow.data(), S, hipMemcpyHostToDevice, stream));
     // Padding rows of both prefix tables stay zero for the life of the solver.
     GSC_HIP(hipMemsetAsync(prefix_valid, 0, PT * sizeof(uint32_t), stream));
     GSC_HIP(hipMemsetAsync(prefix_unreached[0], 0, PT * sizeof(uint32_t), stream));
@@ -327,7 +329,7 @@ void gsc_solver::setup() {
     }
     // Masks on the device (the reference builds them in a serial host loop even in
     // GPU mode, abstraction.hpp:257-265).
-    hipLaunchKernelGGL(masks_kernel, dim3(blocks_for(S, 256)), dim3(256), 0, stream, dc, d_obstacles, d_targets, valid,
+    hipLaunchKernelGGL(masks_kernel, dim3(blocks_for(S, 256)), dim3(256), 0, stream, dc, d_obstacles, d_targets, d_allow, valid,
                        cand, mask_counts);
     GSC_HIP(hipGetLastError());
     unsigned long long* counts = reinterpret_cast<unsigned long long*>(host_words);
@@ -342,8 +344,9 @@ void gsc_solver::setup() {
 
 }
 
-uint64_t gsc_solver::device_bytes(const gsc_case_shape& sh, uint32_t substeps) {
+uint64_t gsc_solver::device_bytes(const gsc_case_shape& sh, const gsc_case_desc& cfg) {
     auto r = [](uint64_t b) { return DeviceArena::round_up(b); };
+    const uint32_t substeps = cfg.integration_substeps;
     const uint64_t S = sh.state_count, PT = sh.prefix_count;
     const uint64_t n0 = sh.state_shape[0], n1 = sh.state_shape[1], n2 = sh.state_shape[2], n3 = sh.state_shape[3];
     const uint64_t na = sh.input_shape[0], nw = sh.input_shape[1], I = sh.input_count;
@@ -358,6 +361,7 @@ uint64_t gsc_solver::device_bytes(const gsc_case_shape& sh, uint32_t substeps) {
     b += r(16 * n2 * nw * K) + r(8 * n3 * na * K) + r(8 * n2 * nw) + r(8 * n3 * na) + r(16 * n3);
     b += r(4 * (n0 + 1) * (n1 + 1)) + r(16 * planes * I * K) + r(32 * planes * I) + 2 * r(64 * 64);
     b += r(32 * planes * I);  // cell offsets
+    if (cfg.state_allow) b += r(S);
     return b;
 }
 

@@ -207,6 +207,7 @@ struct gsc_solver {
     gsc_solver_options opts{};
     std::string name;
     std::vector<gsc_rect4> obstacles, targets;
+    std::vector<uint8_t> allow;  // the caller's per-state constraint mask, when given
     gsc_case_desc cfg{};
     gsc_case_shape shape{};
     DevCase dc{};
@@ -299,7 +300,7 @@ struct gsc_solver {
 
     // Device bytes setup() allocates for a case of this shape (the arena
     // admission estimate; the overflow table may grow later).
-    static uint64_t device_bytes(const gsc_case_shape& sh, uint32_t substeps);
+    static uint64_t device_bytes(const gsc_case_shape& sh, const gsc_case_desc& cfg);
 
     void build_tables(std::vector<double2>& h_cs, std::vector<double>& h_vt, std::vector<uint2>& h_box2,
                       std::vector<uint2>& h_box3, std::vector<double>& h_rad, std::vector<uint32_t>& h_fail);
@@ -325,6 +326,17 @@ inline void copy_case(gsc_solver* s, const gsc_case_desc& c) {
     s->name = c.name ? c.name : "";
     s->obstacles.assign(c.obstacles, c.obstacles + (c.obstacles ? c.obstacle_count : 0));
     s->targets.assign(c.extra_targets, c.extra_targets + (c.extra_targets ? c.extra_target_count : 0));
+    s->allow.clear();
+    if (c.state_allow) {
+        // The caller's mask holds one byte per state; a grid that has no valid
+        // shape fails again, with its message, in setup().
+        try {
+            const gsc_case_shape sh = make_shape(c);
+            s->allow.assign(c.state_allow, c.state_allow + sh.state_count);
+        } catch (const ConfigError&) {
+        }
+    }
+    s->cfg.state_allow = s->allow.empty() ? nullptr : s->allow.data();
     s->cfg.name = s->name.c_str();
     s->cfg.obstacles = s->obstacles.empty() ? nullptr : s->obstacles.data();
     s->cfg.obstacle_count = static_cast<uint32_t>(s->obstacles.size());

@@ -1,26 +1,21 @@
-// backend.cpp — the reference's C++ entry points (config.hpp:69-80,
-// cuda_api.hpp:87-88) implemented over the C ABI of libgsc_amd.so.
-#include <cmath>
-#include <cstdio>
-#include <iomanip>
-#include <sstream>
+// backend.cpp — libgsc_hip.so: the reference's GPU backend symbols
+// (cuda_api.hpp:87-88) with their exact signatures, implemented over the C ABI
+// of libgsc_amd.so. It takes the place of the reference's CUDA library
+// (CMakeLists.txt:24-37, cuda/kernels.cu): a program compiled against the
+// reference's headers links it next to the reference's own config/result code.
+// Built with hidden visibility; only the GSC_EXPORT functions below leave it.
+#include <algorithm>
+#include <cstdint>
 #include <stdexcept>
 
-#include "gsc/cuda_api.hpp"
+#include "gsc/hip_api.hpp"
 #include "gsc_amd.h"
+
+#define GSC_EXPORT __attribute__((visibility("default")))
 
 namespace gsc {
 
 namespace {
-
-HyperRect4D to_rect(const gsc_rect4& r) {
-    HyperRect4D out;
-    for (int d = 0; d < kStateDim; ++d) {
-        out.lb[d] = r.lb[d];
-        out.ub[d] = r.ub[d];
-    }
-    return out;
-}
 
 gsc_rect4 from_rect(const HyperRect4D& r) {
     gsc_rect4 out;
@@ -31,42 +26,10 @@ gsc_rect4 from_rect(const HyperRect4D& r) {
     return out;
 }
 
-CaseConfig from_desc(const gsc_case_desc& d) {
-    CaseConfig c;
-    c.name = d.name ? d.name : "";
-    for (int k = 0; k < kStateDim; ++k) {
-        c.state_lb[k] = d.state_lb[k];
-        c.state_ub[k] = d.state_ub[k];
-        c.state_eta[k] = d.state_eta[k];
-        c.disturbance_half_width[k] = d.disturbance_half_width[k];
-    }
-    c.wrap_dim = d.wrap_dim;
-    for (int k = 0; k < kInputDim; ++k) {
-        c.input_lb[k] = d.input_lb[k];
-        c.input_ub[k] = d.input_ub[k];
-        c.input_eta[k] = d.input_eta[k];
-    }
-    c.candidate = to_rect(d.candidate);
-    c.map = to_rect(d.map);
-    for (uint32_t k = 0; k < d.obstacle_count; ++k) c.obstacles.push_back(to_rect(d.obstacles[k]));
-    c.constraint_type = d.constraint_type == GSC_CONSTRAINT_HYPERBOLIC ? ConstraintType::kHyperbolic
-                        : d.constraint_type == GSC_CONSTRAINT_ELLIPTIC ? ConstraintType::kElliptic
-                                                                       : ConstraintType::kNone;
-    c.hyperbolic_params = {d.hyperbolic_a, d.hyperbolic_b, d.hyperbolic_c};
-    c.elliptic_params = {d.elliptic_a, d.elliptic_b};
-    c.sample_time = d.sample_time;
-    c.integration_substeps = d.integration_substeps;
-    c.max_iterations = d.max_iterations;
-    c.verbose = d.verbose != 0;
-    for (uint32_t k = 0; k < d.extra_target_count; ++k) c.extra_targets.push_back(to_rect(d.extra_targets[k]));
-    c.target_band_enabled = d.target_band_enabled != 0;
-    c.target_band_c = d.target_band_c;
-    c.target_band_w = d.target_band_w;
-    return c;
-}
-
-// The descriptor borrows from `cfg` and `rects`; both must outlive its use.
-gsc_case_desc to_desc(const CaseConfig& cfg, std::vector<gsc_rect4>* obstacles, std::vector<gsc_rect4>* targets) {
+// The descriptor borrows from `cfg`, `ext` and the two vectors; all must
+// outlive its use.
+gsc_case_desc to_desc(const CaseConfig& cfg, const CaseExtensions& ext, std::vector<gsc_rect4>* obstacles,
+                      std::vector<gsc_rect4>* targets) {
     gsc_case_desc d{};
     d.name = cfg.name.c_str();
     for (int k = 0; k < kStateDim; ++k) {
@@ -84,13 +47,13 @@ gsc_case_desc to_desc(const CaseConfig& cfg, std::vector<gsc_rect4>* obstacles, 
     d.candidate = from_rect(cfg.candidate);
     d.map = from_rect(cfg.map);
     for (const auto& r : cfg.obstacles) obstacles->push_back(from_rect(r));
-    for (const auto& r : cfg.extra_targets) targets->push_back(from_rect(r));
+    for (const auto& r : ext.extra_targets) targets->push_back(from_rect(r));
     d.obstacles = obstacles->empty() ? nullptr : obstacles->data();
     d.obstacle_count = static_cast<uint32_t>(obstacles->size());
     d.extra_targets = targets->empty() ? nullptr : targets->data();
     d.extra_target_count = static_cast<uint32_t>(targets->size());
-    // kCustom constraints cannot cross the C ABI; the reference's GPU kernel also
-    // evaluates only none/hyperbolic/elliptic (kernels.cu:55-73).
+    // kCustom crosses as the state_allow mask (run_case_hip_u32); the reference's
+    // GPU kernel evaluates only none/hyperbolic/elliptic (kernels.cu:55-73).
     d.constraint_type = cfg.constraint_type == ConstraintType::kHyperbolic ? GSC_CONSTRAINT_HYPERBOLIC
                         : cfg.constraint_type == ConstraintType::kElliptic ? GSC_CONSTRAINT_ELLIPTIC
                                                                            : GSC_CONSTRAINT_NONE;
@@ -103,91 +66,57 @@ gsc_case_desc to_desc(const CaseConfig& cfg, std::vector<gsc_rect4>* obstacles, 
     d.integration_substeps = cfg.integration_substeps;
     d.max_iterations = cfg.max_iterations;
     d.verbose = cfg.verbose ? 1 : 0;
-    d.target_band_enabled = cfg.target_band_enabled ? 1 : 0;
-    d.target_band_c = cfg.target_band_c;
-    d.target_band_w = cfg.target_band_w;
+    d.target_band_enabled = ext.target_band_enabled ? 1 : 0;
+    d.target_band_c = ext.target_band_c;
+    d.target_band_w = ext.target_band_w;
     return d;
 }
 
-CaseConfig builtin(int which) {
-    gsc_config* h = nullptr;
-    if (gsc_config_builtin(which, &h) != GSC_OK) throw std::runtime_error("unknown canned case");
-    CaseConfig c = from_desc(*gsc_config_desc(h));
-    gsc_config_free(h);
-    return c;
+// kCustom: the reference's mask build evaluates the std::function at every cell
+// centre (abstraction.hpp:97-101, centre as grid.hpp:38-44); the result
+// crosses the C ABI as a per-state byte mask.
+std::vector<std::uint8_t> custom_allow_mask(const CaseConfig& cfg, const gsc_case_shape& sh) {
+    std::vector<std::uint8_t> allow(sh.state_count, 1);
+    if (!cfg.custom_constraint) return allow;
+    std::uint64_t s = 0;
+    for (std::uint32_t i3 = 0; i3 < sh.state_shape[3]; ++i3)
+        for (std::uint32_t i2 = 0; i2 < sh.state_shape[2]; ++i2)
+            for (std::uint32_t i1 = 0; i1 < sh.state_shape[1]; ++i1)
+                for (std::uint32_t i0 = 0; i0 < sh.state_shape[0]; ++i0, ++s) {
+                    const std::uint32_t idx[4] = {i0, i1, i2, i3};
+                    double x[kStateDim];
+                    for (int d = 0; d < kStateDim; ++d)
+                        x[d] = cfg.state_lb[d] + (static_cast<double>(idx[d]) + 0.5) * cfg.state_eta[d];
+                    allow[s] = cfg.custom_constraint(x) ? 1 : 0;
+                }
+    return allow;
 }
 
 }  // namespace
 
-bool point_in_rect(const HyperRect4D& rect, const double x[kStateDim]) {
-    // config.cpp:79-86
-    for (int d = 0; d < kStateDim; ++d)
-        if (x[d] < rect.lb[d] || x[d] > rect.ub[d]) return false;
-    return true;
+GSC_EXPORT bool cuda_backend_compiled() { return gsc_backend_compiled() != 0; }
+
+GSC_EXPORT GpuRunReport run_case_cuda_u32(const CaseConfig& cfg) {
+    return run_case_hip_u32(cfg, CaseExtensions{}, RunOptions{});
 }
 
-bool satisfies_state_constraint(const CaseConfig& cfg, const double x[kStateDim]) {
-    // config.cpp:64-75, 111-127
-    switch (cfg.constraint_type) {
-        case ConstraintType::kHyperbolic: {
-            const double x1_sq = x[0] * x[0];
-            const double x2_sq = x[1] * x[1];
-            return (x1_sq - x2_sq <= cfg.hyperbolic_params.a) &&
-                   (cfg.hyperbolic_params.b * x2_sq - x1_sq <= cfg.hyperbolic_params.c);
-        }
-        case ConstraintType::kElliptic: {
-            const double t1 = (x[0] / cfg.elliptic_params.a) * (x[0] / cfg.elliptic_params.a);
-            const double t2 = (x[1] / cfg.elliptic_params.b) * (x[1] / cfg.elliptic_params.b);
-            return t1 + t2 <= 1.0;
-        }
-        case ConstraintType::kCustom:
-            return cfg.custom_constraint ? cfg.custom_constraint(x) : true;
-        default:
-            return true;
-    }
-}
-
-CaseConfig make_default_case() { return builtin(0); }
-CaseConfig make_wraparound_case() { return builtin(1); }
-CaseConfig make_smoke_case() { return builtin(2); }
-CaseConfig make_hyperbolic_case() { return builtin(3); }
-
-CaseConfig load_case_config(const std::string& path) {
-    gsc_config* h = nullptr;
-    char err[512] = {0};
-    if (gsc_config_load(path.c_str(), &h, err, sizeof(err)) != GSC_OK) throw std::runtime_error(err);
-    CaseConfig c = from_desc(*gsc_config_desc(h));
-    gsc_config_free(h);
-    return c;
-}
-
-std::string format_bytes(std::uint64_t bytes) {
-    // abstraction.hpp:52-63
-    static const char* kUnits[] = {"B", "KiB", "MiB", "GiB", "TiB"};
-    double value = static_cast<double>(bytes);
-    int unit = 0;
-    while (value >= 1024.0 && unit < 4) {
-        value /= 1024.0;
-        ++unit;
-    }
-    std::ostringstream oss;
-    oss << std::fixed << std::setprecision(2) << value << ' ' << kUnits[unit];
-    return oss.str();
-}
-
-bool cuda_backend_compiled() { return gsc_backend_compiled() != 0; }
-
-GpuRunReport run_case_cuda_u32(const CaseConfig& cfg, bool with_abstraction) {
+GSC_EXPORT GpuRunReport run_case_hip_u32(const CaseConfig& cfg, const CaseExtensions& ext, const RunOptions& opts) {
     GpuRunReport report;
     report.compiled_with_cuda = true;
+    const bool with_abstraction = opts.export_abstraction;
     std::vector<gsc_rect4> obstacles, targets;
-    const gsc_case_desc desc = to_desc(cfg, &obstacles, &targets);
+    gsc_case_desc desc = to_desc(cfg, ext, &obstacles, &targets);
 
     gsc_case_shape shape{};
     char err[512] = {0};
     if (gsc_case_shape_of(&desc, &shape, err, sizeof(err)) != GSC_OK) throw std::runtime_error(err);
     const std::uint64_t S = shape.state_count;
     const std::uint64_t P = shape.pair_count;
+    std::vector<std::uint8_t> allow;
+    if (cfg.constraint_type == ConstraintType::kCustom && cfg.custom_constraint) {
+        allow = custom_allow_mask(cfg, shape);
+        desc.state_allow = allow.data();
+    }
 
     auto& res = report.result;
     res.valid_mask.assign(S, 0);

@@ -1,7 +1,11 @@
-// gsc/config.hpp — case configuration, mirroring the reference's public
-// interface (include/gsc/config.hpp:12-82) so callers of the reference compile
-// unchanged against this backend. Parsing and the canned cases are served by
-// the C ABI (gsc_config_load / gsc_config_builtin in libgsc_amd.so).
+// gsc/config.hpp — case configuration with the reference's public interface
+// (include/gsc/config.hpp:12-82): the same members in the same order with the
+// same defaults, so CaseConfig has the reference's layout and a caller
+// compiled against either header passes the same object to
+// gsc::run_case_cuda_u32. Parsing and the canned cases are served by the C ABI
+// (gsc_config_load / gsc_config_builtin in libgsc_amd.so). This backend's
+// extension keys live in gsc::CaseExtensions (gsc/hip_api.hpp), outside
+// CaseConfig.
 #pragma once
 
 #include <cstdint>
@@ -43,8 +47,6 @@ struct EllipticConstraintParams {
 using StateConstraintFn = std::function<bool(const double x[kStateDim])>;
 
 // CaseConfig with the reference's members and defaults (config.hpp:41-67).
-// The last three members are this backend's extensions (extra target boxes
-// and a hyperbolic target band); left empty they reproduce the reference.
 struct CaseConfig {
     std::string name = "unicycle_default";
     double state_lb[kStateDim]{-2.0, -2.0, -3.141592653589793, 0.0};
@@ -63,20 +65,20 @@ struct CaseConfig {
     ConstraintType constraint_type = ConstraintType::kNone;
     HyperbolicConstraintParams hyperbolic_params;
     EllipticConstraintParams elliptic_params;
-    StateConstraintFn custom_constraint;  // not evaluated by the GPU backend (as the reference's)
+    // kCustom: the valid mask takes it at every cell centre; the successor-box
+    // scan does not (the reference's GPU kernel treats type 3 as no constraint,
+    // kernels.cu:55-73, 79).
+    StateConstraintFn custom_constraint;
 
     double disturbance_half_width[kStateDim]{0.05, 0.05, 0.02, 0.02};
     double sample_time = 0.2;
     std::uint32_t integration_substeps = 32;
     std::uint32_t max_iterations = 32;
     bool verbose = true;
-
-    std::vector<HyperRect4D> extra_targets;
-    bool target_band_enabled = false;
-    double target_band_c = 0.0;
-    double target_band_w = 0.0;
 };
 
+// config.cpp:88-109: nullptr for kNone, the parameterised test otherwise.
+StateConstraintFn make_state_constraint(const CaseConfig& cfg);
 bool point_in_rect(const HyperRect4D& rect, const double x[kStateDim]);
 bool satisfies_state_constraint(const CaseConfig& cfg, const double x[kStateDim]);
 CaseConfig make_default_case();

@@ -1,11 +1,19 @@
-// gsc/cuda_api.hpp — the GPU backend boundary with the reference's names
-// (include/gsc/cuda_api.hpp:11-88, reachability.hpp:15-28, grid.hpp:12-79,
-// abstraction.hpp:16-30), implemented over the C ABI of libgsc_amd.so
-// (gsc_run_case_u32). A caller of the reference's run_case_cuda_u32 compiles
-// against this header unchanged.
+// gsc/cuda_api.hpp — the GPU backend boundary with the reference's names and
+// layouts (include/gsc/cuda_api.hpp:11-88, reachability.hpp:15-28,
+// grid.hpp:12-79, abstraction.hpp:16-30), implemented by libgsc_hip.so over
+// the C ABI of libgsc_amd.so (gsc_run_case_u32).
+//
+// Two ways to use it:
+//   - binary drop-in: a program compiled against the reference's own headers
+//     (which also carry its CPU solver) links libgsc_hip.so in place of the
+//     reference's CUDA library; the structs here have the same members in the
+//     same order, so both sides agree on every layout;
+//   - source mirror: a program that only calls run_case_cuda_u32, the config
+//     functions and the HDF5 writer compiles against these headers.
 #pragma once
 
 #include <cstdint>
+#include <cstdio>
 #include <string>
 #include <vector>
 
@@ -135,14 +143,28 @@ struct GpuRunReport {
     std::vector<std::uint8_t> abstraction_valid;
 };
 
+// The two backend symbols (cuda_api.hpp:87-88), exported by libgsc_hip.so with
+// the reference's exact signatures. Like the reference, preparation errors
+// (grid shape, candidate outside the valid set, ...) are thrown as
+// std::runtime_error, and device errors come back in report.message with
+// executed == false. The report always carries the per-pair abstraction
+// triplet, as the reference's does; gsc::run_case_hip_u32 (gsc/hip_api.hpp)
+// can leave it out.
 bool cuda_backend_compiled();
-// Like the reference: preparation errors (grid shape, candidate outside the
-// valid set, ...) are thrown as std::runtime_error; device errors come back
-// in report.message with executed == false. `with_abstraction` = false skips
-// the pair-abstraction copy to the host (3.6 GB at 401M pairs), which the
-// reference always performs to fill /abstraction in its HDF5 output.
-GpuRunReport run_case_cuda_u32(const CaseConfig& cfg, bool with_abstraction = true);
+GpuRunReport run_case_cuda_u32(const CaseConfig& cfg);
 
-std::string format_bytes(std::uint64_t bytes);
+// abstraction.hpp:52-63 (inline there as well).
+inline std::string format_bytes(std::uint64_t bytes) {
+    static const char* kUnits[] = {"B", "KiB", "MiB", "GiB", "TiB"};
+    double value = static_cast<double>(bytes);
+    int unit = 0;
+    while (value >= 1024.0 && unit < 4) {
+        value /= 1024.0;
+        ++unit;
+    }
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "%.2f %s", value, kUnits[unit]);
+    return buf;
+}
 
 }  // namespace gsc

@@ -15,7 +15,7 @@
 #include <stdexcept>
 #include <string>
 
-#include "gsc/cuda_api.hpp"
+#include "gsc/hip_api.hpp"
 #include "gsc/result_io.hpp"
 
 namespace {
@@ -115,7 +115,8 @@ int main(int argc, char** argv) {
             }
             throw std::runtime_error("unknown argument: " + arg);
         }
-        const auto cfg = gsc::load_case_config(config_path);
+        gsc::CaseExtensions ext;
+        const auto cfg = gsc::load_case_config_ext(config_path, &ext);
         const std::string mode_name = mode.substr(2);
         if (output_path.empty()) output_path = gsc::default_result_path(cfg, mode_name);
 
@@ -123,7 +124,9 @@ int main(int argc, char** argv) {
             throw std::runtime_error("mode " + mode + " runs the reference's CPU solver, which is not part of this "
                                      "backend; use --gpu");
 
-        const auto gpu = gsc::run_case_cuda_u32(cfg, export_abstraction);
+        gsc::RunOptions run_opts;
+        run_opts.export_abstraction = export_abstraction;
+        const auto gpu = gsc::run_case_hip_u32(cfg, ext, run_opts);
         if (!gpu.executed) {
             std::cerr << "gpu run unavailable: " << gpu.message << '\n';
             return 2;

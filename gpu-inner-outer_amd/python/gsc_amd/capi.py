@@ -64,6 +64,7 @@ class CaseDesc(ctypes.Structure):
         ("target_band_enabled", c_int32),
         ("target_band_c", c_double),
         ("target_band_w", c_double),
+        ("state_allow", POINTER(ctypes.c_uint8)),
     ]
 
 

@@ -59,12 +59,20 @@ typedef struct gsc_rect4 {
 
 /*
  * CaseConfig, config.hpp:41-67, as a POD. Arrays are borrowed for the duration
- * of a call. The three trailing extension fields have no reference
- * counterpart; left zero they reproduce the reference exactly:
+ * of a call. The trailing extension fields have no reference counterpart;
+ * left zero they reproduce the reference exactly:
  *   extra_targets   : further target boxes; a state is a candidate if its
  *                     centre lies in `candidate` OR in any extra target box.
  *   target_band_*   : when enabled, a centre in `candidate` only counts if
  *                     fabs(x*y - c) <= w as well (a hyperbolic band in (x, y)).
+ *   state_allow     : NULL, or state_count bytes (x-fastest flat order); a 0
+ *                     removes that state from the valid set. It carries a
+ *                     caller-side pointwise constraint (CaseConfig's kCustom
+ *                     std::function, config.hpp:38, 63) that cannot cross a C
+ *                     ABI: the reference evaluates it in its host mask build
+ *                     (abstraction.hpp:97-101) and its GPU box scan treats it
+ *                     as no constraint (kernels.cu:55-73, 79), as this backend
+ *                     does with constraint_type NONE plus this mask.
  */
 typedef struct gsc_case_desc {
     const char* name;
@@ -93,6 +101,7 @@ typedef struct gsc_case_desc {
     int32_t target_band_enabled;
     double target_band_c;
     double target_band_w;
+    const uint8_t* state_allow;
 } gsc_case_desc;
 
 /* Grid4D / InputGrid2D / PrefixLayout4D sizes (grid.hpp:12-101, 103-168, 253-269). */

@@ -8,6 +8,10 @@ For every case, oracle/_ref/ref_harness (load_case_config -> prepare_case_cpu
 fixture records SHA-256 digests of every output array, the per-iteration
 counts and the stop reason. Tiny cases also keep their full arrays (.npz).
 `--large` adds the 401,310,000-pair cases (several minutes each).
+`--batch` and `--serving` write digests and per-iteration counts for every
+case of the full-size batching workload (32 cases of 5e4-1e6 cells) and the
+serving workload (48 requests of 2e4-4e6 cells), several reference processes
+at a time (`--jobs`); with `--only NAME` the small fixtures are left as they are.
 
 The published case is additionally tied to the two artefacts the reference
 ships: the README run (README.md:44-98: counts per iteration) and the
@@ -73,6 +77,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--large", action="store_true")
     ap.add_argument("--only", default=None)
+    ap.add_argument("--batch", action="store_true", help="the 32 full-size batching cases")
+    ap.add_argument("--serving", action="store_true", help="the 48 full-size serving requests")
+    ap.add_argument("--jobs", type=int, default=6, help="reference processes at a time for --batch / --serving")
     args = ap.parse_args()
     if not gsc_oracle.ref_available():
         raise SystemExit("build the reference harness first: make -C oracle ref")
@@ -132,6 +139,33 @@ def main():
         fx = fixture(case, ref)
         fx["reference_cpu_ms"] = {"abstraction": ref["abstraction_ms"], "solve": ref["solve_ms"]}
         write("unicycle_91x105x42x10", fx)
+
+    # Full-size batching and serving workloads: every case of
+    # batch_instances(32, seed=3) (5e4-1e6 cells) and of serving_requests(48,
+    # seed=5) (2e4-4e6 cells), each solved by the reference build on its own.
+    if args.batch and (not args.only or args.only == "batch_instances_32_s3"):
+        write("batch_instances_32_s3", many_fixtures(cases.batch_instances(32, seed=3), args.jobs))
+    if args.serving and (not args.only or args.only == "serving_requests_48_s5"):
+        write("serving_requests_48_s5", many_fixtures(cases.serving_requests(48, seed=5), args.jobs))
+
+
+def _one_fixture(case):
+    ref = gsc_oracle.run_ref(case, timeout=4 * 3600)
+    fx = fixture(case, ref)
+    fx["reference_cpu_ms"] = {"abstraction": ref["abstraction_ms"], "solve": ref["solve_ms"]}
+    print(f"  {case.name}: {ref['iterations']} iterations, {ref['message']}", flush=True)
+    return case.name, fx
+
+
+def many_fixtures(case_list, jobs):
+    """Reference solves of many cases, one single-threaded reference process per
+    case, `jobs` at a time; largest first so the long ones start early."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    order = sorted(case_list, key=lambda c: -c.state_count() * c.input_count())
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        done = dict(ex.map(_one_fixture, order))
+    return {c.name: done[c.name] for c in case_list}
 
 
 if __name__ == "__main__":
