@@ -52,8 +52,20 @@ struct gsc_batch {
     uint32_t plane_lds = 0;
     std::vector<void*> device_allocs;
 
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // One captured, instantiated graph per (tile_wg, tile_ib, iterations per
+    // launch), kept until the batch is destroyed. Its nodes read the instance
+    // table and block ranges from d_inst / d_starts at run time, and its grid
+    // sizes depend only on the tile width (the per-stage totals do not depend
+    // on the case order), so neither a re-plan nor a new abstraction outdates
+    // it. Tuning therefore never destroys and re-instantiates graphs while the
+    // batch lives.
+    struct GraphEntry {
+        int32_t wg, ib, g;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
+    hipGraphExec_t exec = nullptr;  // the entry of the current options, once captured
     bool planned = false;
     bool abstraction_ready = false;
     bool solved = false;
@@ -65,7 +77,7 @@ struct gsc_batch {
 
     ~gsc_batch() {
         if (stream) (void)hipStreamSynchronize(stream);
-        drop_graph();
+        drop_graphs();
         for (void* p : device_allocs) (void)hipFree(p);
         if (h_status) (void)hipHostFree(h_status);
         if (h_records) (void)hipHostFree(h_records);
@@ -85,11 +97,21 @@ struct gsc_batch {
         return static_cast<T*>(p);
     }
 
-    void drop_graph() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
+    // Destructor only, after the stream has drained.
+    void drop_graphs() {
+        for (GraphEntry& e : graphs) {
+            if (e.exec) (void)hipGraphExecDestroy(e.exec);
+            if (e.graph) (void)hipGraphDestroy(e.graph);
+        }
+        graphs.clear();
         exec = nullptr;
-        graph = nullptr;
+    }
+
+    // The cached graph of the current options; nullptr when not captured yet.
+    hipGraphExec_t find_exec() const {
+        for (const GraphEntry& e : graphs)
+            if (e.wg == opts.tile_wg && e.ib == opts.tile_ib && e.g == opts.iterations_per_launch) return e.exec;
+        return nullptr;
     }
 
     void create(const gsc_case_desc* const* descs, uint32_t count);
@@ -228,7 +250,6 @@ void gsc_batch::plan() {
     GSC_HIP(hipMemcpyAsync(d_inst, h_inst.data(), sizeof(BatchInstance) * N, hipMemcpyHostToDevice, stream));
     GSC_HIP(hipMemcpyAsync(d_starts, starts.data(), sizeof(uint32_t) * starts.size(), hipMemcpyHostToDevice, stream));
     GSC_HIP(hipStreamSynchronize(stream));
-    drop_graph();
     planned = true;
 }
 
@@ -274,6 +295,7 @@ void gsc_batch::solve() {
     if (!planned) plan();
     const uint32_t N = n();
     const uint32_t G = static_cast<uint32_t>(opts.iterations_per_launch);
+    exec = opts.use_graph ? find_exec() : nullptr;
     if (opts.use_graph && !exec) {
         GSC_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         try {
@@ -284,8 +306,15 @@ void gsc_batch::solve() {
             if (g) (void)hipGraphDestroy(g);
             throw;
         }
+        hipGraph_t graph = nullptr;
         GSC_HIP(hipStreamEndCapture(stream, &graph));
-        GSC_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            exec = nullptr;
+            throw DeviceError(e, hipGetErrorString(e));
+        }
+        graphs.push_back(GraphEntry{opts.tile_wg, opts.tile_ib, opts.iterations_per_launch, graph, exec});
     }
 
     const auto t0 = std::chrono::steady_clock::now();
@@ -454,13 +483,11 @@ int gsc_batch_set_options(gsc_batch* b, const gsc_batch_options* opts) {
     if (!b || !opts) return GSC_ERR_INVALID_ARGUMENT;
     if (!options_valid(*opts) || opts->device != b->opts.device) return GSC_ERR_INVALID_ARGUMENT;
     return guarded(b, [&] {
+        // A new tile width or order needs new block ranges; the graph of the new
+        // options is looked up (or captured) by the next solve.
         const bool replan = opts->tile_wg != b->opts.tile_wg || opts->order != b->opts.order;
-        const bool recapture = replan || opts->tile_ib != b->opts.tile_ib ||
-                               opts->iterations_per_launch != b->opts.iterations_per_launch ||
-                               opts->use_graph != b->opts.use_graph;
         b->opts = *opts;
         if (replan) b->planned = false;
-        if (recapture) b->drop_graph();
     });
 }
 

@@ -300,13 +300,34 @@ void gsc_server::admit(std::vector<Request*>& admitted, std::vector<Request*>& p
     for (Request* r : launched) {
         gsc_solver& s = *r->solver;
         try {
+            // May regrow the overflow table from the arena (more escaped boxes
+            // than the first table held); admission counts the arena's bytes in
+            // use, so the regrown table is charged like every other buffer.
             s.finish_abstraction();
             s.seed_async();
+            {
+                std::lock_guard<std::mutex> lock(m);
+                r->est_bytes = s.bytes_allocated;  // what the request holds in the arena now
+            }
             prepared.push_back(r);
         } catch (const ConfigError& e) {
             std::lock_guard<std::mutex> lock(m);
+            r->solver.reset();
             release_slot(r);
             fail(r, GSC_ERR_CONFIG, e.what());
+        } catch (const DeviceError& e) {
+            // The regrown table did not fit: like a set-up that did not fit, the
+            // request waits for running cases to leave, or fails alone.
+            const bool oom = status_of(e) == GSC_ERR_OUT_OF_MEMORY;
+            std::lock_guard<std::mutex> lock(m);
+            r->solver.reset();
+            release_slot(r);
+            if (oom && !active.empty()) {
+                r->state = GSC_REQUEST_QUEUED;
+                queue.push_front(r);
+            } else {
+                fail(r, status_of(e), e.what());
+            }
         }
     }
     GSC_HIP(hipStreamSynchronize(prep_stream));

@@ -202,7 +202,14 @@ void gsc_solver::setup() {
     dc.n_targets = static_cast<uint32_t>(targets.size());
 
     GSC_HIP(hipSetDevice(opts.device));
-    if (!stream) {
+    if (no_alloc) {
+        // The null stream, no events, no pinned memory: nothing is created.
+        if (!arena || !host_words || !host_ctl || !host_records)
+            throw ConfigError("a solver without allocation needs an arena and lent host words");
+        stream = nullptr;
+        owns_stream = false;
+        opts.time_kernels = 0;
+    } else if (!stream) {
         GSC_HIP(hipStreamCreate(&stream));
         owns_stream = true;
     }
@@ -237,41 +244,25 @@ void gsc_solver::setup() {
         owns_host_words = true;
     }
     std::memset(host_words, 0, 4 * sizeof(uint32_t));
-    events.resize(opts.time_kernels ? kBatch * kEventsPerIter + 2 : 2);
-    for (auto& e : events) GSC_HIP(hipEventCreate(&e));
-
-    std::vector<double2> h_cs;
-    std::vector<double> h_vt, h_rad;
-    std::vector<uint2> h_box2, h_box3;
-    std::vector<uint32_t> h_fail;
-    build_tables(h_cs, h_vt, h_box2, h_box3, h_rad, h_fail);
-    {
-        // Filter constants of the x / y fast path (set_cell_filter): every cell
-        // centre lies within [lb, lb + n eta + eta], every increment within
-        // |dt| max|v| (times three roundings), every radius within the table's.
-        double vmax = 0.0, rmax[2] = {0.0, 0.0};
-        for (const double v : h_vt) vmax = std::max(vmax, std::fabs(v));
-        for (size_t k = 0; k + 1 < h_rad.size(); k += 2) {
-            rmax[0] = std::max(rmax[0], std::fabs(h_rad[k]));
-            rmax[1] = std::max(rmax[1], std::fabs(h_rad[k + 1]));
-        }
-        const double abs_sum = static_cast<double>(dc.K) * std::fabs(dc.dt) * vmax * (1.0 + 1e-12);
-        for (int d = 0; d < 2; ++d) {
-            const double top = dc.lb[d] + static_cast<double>(dc.shape[d]) * dc.eta[d];
-            const double x_abs = std::max(std::fabs(dc.lb[d]), std::fabs(top)) + std::fabs(dc.eta[d]);
-            set_cell_filter(dc, d, dc.K, x_abs, abs_sum, rmax[d]);
-        }
+    if (!no_alloc) {
+        events.resize(opts.time_kernels ? kBatch * kEventsPerIter + 2 : 2);
+        for (auto& e : events) GSC_HIP(hipEventCreate(&e));
     }
-    cs = alloc<double2>(h_cs.size());
-    vt = alloc<double>(h_vt.size());
-    box2 = alloc<uint2>(h_box2.size());
-    box3 = alloc<uint2>(h_box3.size());
-    const std::vector<uint2> h_hull2 = cell_hulls(h_box2, dc.shape[2], dc.n_w, dc.wrap_dim == 2);
-    const std::vector<uint2> h_hull3 = cell_hulls(h_box3, dc.shape[3], dc.n_a, dc.wrap_dim == 3);
-    hull2 = alloc<uint2>(h_hull2.size());
-    hull3 = alloc<uint2>(h_hull3.size());
-    rad = alloc<double>(h_rad.size());
-    if (!h_fail.empty()) fail2d = alloc<uint32_t>(h_fail.size());
+
+    // The model tables (trigonometry, θ / v boxes, radii, constraint table,
+    // trajectory increments) are filled by every launch_abstraction: they are
+    // part of the abstraction's work, which the reference does inside its
+    // timed kernel (kernels.cu:164-217, 609-641). Sizes as in build_tables.
+    const uint64_t n2 = dc.shape[2], n3 = dc.shape[3];
+    cs = alloc<double2>(n2 * dc.n_w * dc.K);
+    vt = alloc<double>(n3 * dc.n_a * dc.K);
+    box2 = alloc<uint2>(n2 * dc.n_w);
+    box3 = alloc<uint2>(n3 * dc.n_a);
+    hull2 = alloc<uint2>(n2);
+    hull3 = alloc<uint2>(n3);
+    rad = alloc<double>(2 * n3);
+    if (cfg.constraint_type != GSC_CONSTRAINT_NONE)
+        fail2d = alloc<uint32_t>((static_cast<uint64_t>(dc.shape[1]) + 1) << dc.fail_shift);
     const uint64_t n_inc = static_cast<uint64_t>(dc.shape[2]) * dc.shape[3] * dc.I * dc.K;
     if (n_inc * sizeof(double2) > (4ull << 30)) throw ConfigError("trajectory increment table exceeds 4 GiB");
     inc = alloc<double2>(n_inc);
@@ -282,15 +273,6 @@ void gsc_solver::setup() {
     uint8_t* d_allow = allow.empty() ? nullptr : alloc<uint8_t>(S);
 
     const auto t0 = std::chrono::steady_clock::now();
-    GSC_HIP(hipMemcpyAsync(cs, h_cs.data(), h_cs.size() * sizeof(double2), hipMemcpyHostToDevice, stream));
-    GSC_HIP(hipMemcpyAsync(vt, h_vt.data(), h_vt.size() * sizeof(double), hipMemcpyHostToDevice, stream));
-    GSC_HIP(hipMemcpyAsync(box2, h_box2.data(), h_box2.size() * sizeof(uint2), hipMemcpyHostToDevice, stream));
-    GSC_HIP(hipMemcpyAsync(box3, h_box3.data(), h_box3.size() * sizeof(uint2), hipMemcpyHostToDevice, stream));
-    GSC_HIP(hipMemcpyAsync(hull2, h_hull2.data(), h_hull2.size() * sizeof(uint2), hipMemcpyHostToDevice, stream));
-    GSC_HIP(hipMemcpyAsync(hull3, h_hull3.data(), h_hull3.size() * sizeof(uint2), hipMemcpyHostToDevice, stream));
-    GSC_HIP(hipMemcpyAsync(rad, h_rad.data(), h_rad.size() * sizeof(double), hipMemcpyHostToDevice, stream));
-    if (fail2d)
-        GSC_HIP(hipMemcpyAsync(fail2d, h_fail.data(), h_fail.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
     if (!obstacles.empty())
         GSC_HIP(hipMemcpyAsync(d_obstacles, obstacles.data(), obstacles.size() * sizeof(gsc_rect4),
                                hipMemcpyHostToDevice, stream));
@@ -313,20 +295,6 @@ void gsc_solver::setup() {
     }
     GSC_HIP(hipMemsetAsync(mask_counts, 0, 2 * sizeof(unsigned long long), stream));
 
-    hipLaunchKernelGGL(probe_zw_kernel, dim3(blocks_for(n_planes * dc.I_pad, 256)), dim3(256), 0, stream, dc, box2,
-                       box3, probe_zw);
-    GSC_HIP(hipGetLastError());
-    if (n_inc) {
-        hipLaunchKernelGGL(increments_kernel, dim3(blocks_for(n_inc, 256)), dim3(256), 0, stream, dc, cs, vt, inc);
-        GSC_HIP(hipGetLastError());
-    }
-    {
-        // Also with no substeps (K = 0): the sums are then zero.
-        const uint64_t n_rows = static_cast<uint64_t>(dc.shape[2]) * dc.shape[3] * dc.I;
-        hipLaunchKernelGGL(increment_sums_kernel, dim3(blocks_for(n_rows, 256)), dim3(256), 0, stream, dc, inc, rad,
-                           inc_sums, cell_offs);
-        GSC_HIP(hipGetLastError());
-    }
     // Masks on the device (the reference builds them in a serial host loop even in
     // GPU mode, abstraction.hpp:257-265).
     hipLaunchKernelGGL(masks_kernel, dim3(blocks_for(S, 256)), dim3(256), 0, stream, dc, d_obstacles, d_targets, d_allow, valid,
@@ -398,8 +366,64 @@ void gsc_solver::build_abstraction() {
     finish_abstraction();
 }
 
+// The model tables of the abstraction: built on the host (trigonometry with
+// the host libm, θ / v boxes, radii, constraint table), uploaded, then the
+// per-(plane, input) tables on the device (probe offsets, trajectory
+// increments and their sums). The host vectors stay alive in `tables` until
+// the next call, past the asynchronous copies.
+void gsc_solver::upload_tables() {
+    HostTables& t = tables;
+    build_tables(t.cs, t.vt, t.box2, t.box3, t.rad, t.fail);
+    t.hull2 = cell_hulls(t.box2, dc.shape[2], dc.n_w, dc.wrap_dim == 2);
+    t.hull3 = cell_hulls(t.box3, dc.shape[3], dc.n_a, dc.wrap_dim == 3);
+    {
+        // Filter constants of the x / y fast path (set_cell_filter): every cell
+        // centre lies within [lb, lb + n eta + eta], every increment within
+        // |dt| max|v| (times three roundings), every radius within the table's.
+        double vmax = 0.0, rmax[2] = {0.0, 0.0};
+        for (const double v : t.vt) vmax = std::max(vmax, std::fabs(v));
+        for (size_t k = 0; k + 1 < t.rad.size(); k += 2) {
+            rmax[0] = std::max(rmax[0], std::fabs(t.rad[k]));
+            rmax[1] = std::max(rmax[1], std::fabs(t.rad[k + 1]));
+        }
+        const double abs_sum = static_cast<double>(dc.K) * std::fabs(dc.dt) * vmax * (1.0 + 1e-12);
+        for (int d = 0; d < 2; ++d) {
+            const double top = dc.lb[d] + static_cast<double>(dc.shape[d]) * dc.eta[d];
+            const double x_abs = std::max(std::fabs(dc.lb[d]), std::fabs(top)) + std::fabs(dc.eta[d]);
+            set_cell_filter(dc, d, dc.K, x_abs, abs_sum, rmax[d]);
+        }
+    }
+    if (!t.cs.empty())
+        GSC_HIP(hipMemcpyAsync(cs, t.cs.data(), t.cs.size() * sizeof(double2), hipMemcpyHostToDevice, stream));
+    if (!t.vt.empty())
+        GSC_HIP(hipMemcpyAsync(vt, t.vt.data(), t.vt.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    GSC_HIP(hipMemcpyAsync(box2, t.box2.data(), t.box2.size() * sizeof(uint2), hipMemcpyHostToDevice, stream));
+    GSC_HIP(hipMemcpyAsync(box3, t.box3.data(), t.box3.size() * sizeof(uint2), hipMemcpyHostToDevice, stream));
+    GSC_HIP(hipMemcpyAsync(hull2, t.hull2.data(), t.hull2.size() * sizeof(uint2), hipMemcpyHostToDevice, stream));
+    GSC_HIP(hipMemcpyAsync(hull3, t.hull3.data(), t.hull3.size() * sizeof(uint2), hipMemcpyHostToDevice, stream));
+    GSC_HIP(hipMemcpyAsync(rad, t.rad.data(), t.rad.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    if (fail2d)
+        GSC_HIP(hipMemcpyAsync(fail2d, t.fail.data(), t.fail.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    const uint64_t n_planes = static_cast<uint64_t>(dc.shape[2]) * dc.shape[3];
+    hipLaunchKernelGGL(probe_zw_kernel, dim3(blocks_for(n_planes * dc.I_pad, 256)), dim3(256), 0, stream, dc, box2,
+                       box3, probe_zw);
+    GSC_HIP(hipGetLastError());
+    const uint64_t n_inc = n_planes * dc.I * dc.K;
+    if (n_inc) {
+        hipLaunchKernelGGL(increments_kernel, dim3(blocks_for(n_inc, 256)), dim3(256), 0, stream, dc, cs, vt, inc);
+        GSC_HIP(hipGetLastError());
+    }
+    // Also with no substeps (K = 0): the sums are then zero.
+    const uint64_t n_rows = n_planes * dc.I;
+    hipLaunchKernelGGL(increment_sums_kernel, dim3(blocks_for(n_rows, 256)), dim3(256), 0, stream, dc, inc, rad,
+                       inc_sums, cell_offs);
+    GSC_HIP(hipGetLastError());
+}
+
 void gsc_solver::launch_abstraction() {
     abstraction_t0 = std::chrono::steady_clock::now();
+    if (!events.empty()) GSC_HIP(hipEventRecord(events[0], stream));
+    upload_tables();
     // A fill kernel, not hipMemsetAsync, so a caller may capture this sequence.
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, overflow_count, 1u, 0u);
     GSC_HIP(hipGetLastError());
@@ -419,10 +443,9 @@ void gsc_solver::launch_abstraction() {
     A.overflow_capacity = overflow_capacity;
     A.hull = hull;
     A.chunk_hull = chunk_hull;
-    GSC_HIP(hipEventRecord(events[0], stream));
     hipLaunchKernelGGL(abstraction_kernel, dim3(blocks_for(dc.n_tasks, 4)), dim3(256), 0, stream, A);
     GSC_HIP(hipGetLastError());
-    GSC_HIP(hipEventRecord(events[1], stream));
+    if (!events.empty()) GSC_HIP(hipEventRecord(events[1], stream));
     GSC_HIP(hipMemcpyAsync(host_words, overflow_count, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
 }
 
@@ -432,6 +455,9 @@ void gsc_solver::finish_abstraction() {
     if (used > overflow_capacity) {
         // More escaped boxes than the table holds: grow it and run once more.
         if (used >= kRecIndexMask) throw ConfigError("too many irregular successor boxes");
+        // The old table goes first, so an arena can reuse its bytes.
+        release(overflow);
+        overflow = nullptr;
         overflow_capacity = used;
         overflow = alloc<OverflowEntry>(overflow_capacity);
         const auto t0 = abstraction_t0;
@@ -441,10 +467,14 @@ void gsc_solver::finish_abstraction() {
         used = host_words[0];
         if (used > overflow_capacity) throw ConfigError("successor-box overflow table did not converge");
     }
-    float ms = 0.f;
-    GSC_HIP(hipEventElapsedTime(&ms, events[0], events[1]));
-    abstraction_kernel_ms = ms;
     abstraction_wall_ms = ms_since(abstraction_t0);
+    if (events.empty()) {
+        abstraction_kernel_ms = abstraction_wall_ms;  // no events without allocation: wall time
+    } else {
+        float ms = 0.f;
+        GSC_HIP(hipEventElapsedTime(&ms, events[0], events[1]));
+        abstraction_kernel_ms = ms;
+    }
     abstraction_ready = true;
 }
 
@@ -456,6 +486,10 @@ void gsc_solver::solve() {
     }
     if (!host_records) {
         GSC_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_records), sizeof(IterRecord) * kRing, hipHostMallocDefault));
+        std::memset(host_records, 0, sizeof(IterRecord) * kRing);
+    }
+    if (no_alloc) {
+        std::memset(host_ctl, 0, sizeof(IterCtl));
         std::memset(host_records, 0, sizeof(IterRecord) * kRing);
     }
     const uint32_t S = dc.S;
@@ -619,6 +653,97 @@ void gsc_solver::solve() {
     solved = true;
 }
 
+// Host copy of n bytes split over a few threads: the staging buffer is pinned
+// and the destination is the caller's pageable memory, so one thread alone is
+// bound by its own memcpy bandwidth, well below the link's.
+static void parallel_copy(void* dst, const void* src, uint64_t n) {
+    constexpr unsigned kThreads = 4;
+    if (n < (8u << 20)) {
+        std::memcpy(dst, src, n);
+        return;
+    }
+    const uint64_t part = (n / kThreads + 4095) & ~uint64_t(4095);
+    std::thread t[kThreads - 1];
+    for (unsigned k = 1; k < kThreads; ++k) {
+        const uint64_t b = std::min(n, part * k), e = std::min(n, part * (k + 1));
+        t[k - 1] = std::thread([=] { std::memcpy(static_cast<char*>(dst) + b, static_cast<const char*>(src) + b, e - b); });
+    }
+    std::memcpy(dst, src, std::min(n, part));
+    for (auto& th : t) th.join();
+}
+
+// The reference's abstraction triplet (kernels.cu:806-824) for every pair, in
+// slices: export_kernel expands a slice of records on the device, an async
+// copy moves it into one of two pinned host buffers, and while the host copies
+// that buffer into the caller's arrays the device already works on the next
+// slice into the other pair of buffers. Staging is allocated once per solver
+// and kept for later calls. A solver without allocation (no_alloc) takes
+// its device staging from its arena and copies each slice straight into the
+// caller's arrays.
+void gsc_solver::export_triplet(const gsc_result_buffers* out) {
+    const uint64_t P = shape.pair_count;
+    const bool want[3] = {out->abstraction_min_flat != nullptr, out->abstraction_max_flat != nullptr,
+                          out->abstraction_valid != nullptr};
+    if (!export_slice) {
+        export_slice = std::min<uint64_t>(P, no_alloc ? kExportSliceNoAlloc : kExportSlice);
+        for (int k = 0; k < (no_alloc ? 1 : 2); ++k) {
+            ExportStage& st = xstage[k];
+            st.dmin = alloc<uint32_t>(export_slice);
+            st.dmax = alloc<uint32_t>(export_slice);
+            st.dval = alloc<uint8_t>(export_slice);
+            if (no_alloc) continue;
+            GSC_HIP(hipHostMalloc(reinterpret_cast<void**>(&st.hmin), export_slice * 4, hipHostMallocDefault));
+            GSC_HIP(hipHostMalloc(reinterpret_cast<void**>(&st.hmax), export_slice * 4, hipHostMallocDefault));
+            GSC_HIP(hipHostMalloc(reinterpret_cast<void**>(&st.hval), export_slice, hipHostMallocDefault));
+            GSC_HIP(hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
+        }
+    }
+    const uint64_t n = (P + export_slice - 1) / export_slice;
+    auto expand = [&](uint64_t k, ExportStage& st) {
+        const uint64_t b = k * export_slice, e = std::min(P, b + export_slice);
+        hipLaunchKernelGGL(export_kernel, dim3(blocks_for(e - b, 256)), dim3(256), 0, stream, dc, rec, box2, box3,
+                           overflow, b, e, st.dmin, st.dmax, st.dval);
+        GSC_HIP(hipGetLastError());
+    };
+    if (no_alloc) {
+        for (uint64_t k = 0; k < n; ++k) {
+            const uint64_t b = k * export_slice, e = std::min(P, b + export_slice);
+            expand(k, xstage[0]);
+            if (want[0])
+                GSC_HIP(hipMemcpyAsync(out->abstraction_min_flat + b, xstage[0].dmin, (e - b) * 4,
+                                       hipMemcpyDeviceToHost, stream));
+            if (want[1])
+                GSC_HIP(hipMemcpyAsync(out->abstraction_max_flat + b, xstage[0].dmax, (e - b) * 4,
+                                       hipMemcpyDeviceToHost, stream));
+            if (want[2])
+                GSC_HIP(hipMemcpyAsync(out->abstraction_valid + b, xstage[0].dval, e - b, hipMemcpyDeviceToHost, stream));
+            GSC_HIP(hipStreamSynchronize(stream));
+        }
+        return;
+    }
+    auto enqueue = [&](uint64_t k) {
+        ExportStage& st = xstage[k & 1];
+        const uint64_t m = std::min(P, (k + 1) * export_slice) - k * export_slice;
+        expand(k, st);
+        if (want[0]) GSC_HIP(hipMemcpyAsync(st.hmin, st.dmin, m * 4, hipMemcpyDeviceToHost, stream));
+        if (want[1]) GSC_HIP(hipMemcpyAsync(st.hmax, st.dmax, m * 4, hipMemcpyDeviceToHost, stream));
+        if (want[2]) GSC_HIP(hipMemcpyAsync(st.hval, st.dval, m, hipMemcpyDeviceToHost, stream));
+        GSC_HIP(hipEventRecord(st.done, stream));
+    };
+    enqueue(0);
+    if (n > 1) enqueue(1);
+    for (uint64_t k = 0; k < n; ++k) {
+        ExportStage& st = xstage[k & 1];
+        GSC_HIP(hipEventSynchronize(st.done));
+        const uint64_t b = k * export_slice, m = std::min(P, b + export_slice) - b;
+        if (want[0]) parallel_copy(out->abstraction_min_flat + b, st.hmin, m * 4);
+        if (want[1]) parallel_copy(out->abstraction_max_flat + b, st.hmax, m * 4);
+        if (want[2]) parallel_copy(out->abstraction_valid + b, st.hval, m);
+        // This pair of buffers is free again: slice k + 2 may overwrite it.
+        if (k + 2 < n) enqueue(k + 2);
+    }
+}
+
 void gsc_solver::download(const gsc_result_buffers* out, gsc_run_report* report) {
     const auto t0 = std::chrono::steady_clock::now();
     const uint64_t S = shape.state_count;
@@ -636,40 +761,9 @@ void gsc_solver::download(const gsc_result_buffers* out, gsc_run_report* report)
         if (abstraction_ready && (out->abstraction_min_flat || out->abstraction_max_flat || out->abstraction_valid)) {
             if (opts.recompute_abstraction)
                 throw ConfigError("the pair abstraction is not stored in recompute mode");
-            // Expand the records to the reference triplet in slices through device staging.
-            const uint64_t P = shape.pair_count;
-            const uint64_t slice = std::min<uint64_t>(P, 1ull << 26);
-            uint32_t* dmin = nullptr;
-            uint32_t* dmax = nullptr;
-            uint8_t* dval = nullptr;
-            GSC_HIP(hipMalloc(&dmin, slice * 4));
-            GSC_HIP(hipMalloc(&dmax, slice * 4));
-            GSC_HIP(hipMalloc(&dval, slice));
-            try {
-                for (uint64_t b = 0; b < P; b += slice) {
-                    const uint64_t e = std::min(P, b + slice);
-                    hipLaunchKernelGGL(export_kernel, dim3(blocks_for(e - b, 256)), dim3(256), 0, stream, dc, rec, box2,
-                                       box3, overflow, b, e, dmin, dmax, dval);
-                    GSC_HIP(hipGetLastError());
-                    if (out->abstraction_min_flat)
-                        GSC_HIP(hipMemcpyAsync(out->abstraction_min_flat + b, dmin, (e - b) * 4, hipMemcpyDeviceToHost,
-                                               stream));
-                    if (out->abstraction_max_flat)
-                        GSC_HIP(hipMemcpyAsync(out->abstraction_max_flat + b, dmax, (e - b) * 4, hipMemcpyDeviceToHost,
-                                               stream));
-                    if (out->abstraction_valid)
-                        GSC_HIP(hipMemcpyAsync(out->abstraction_valid + b, dval, e - b, hipMemcpyDeviceToHost, stream));
-                    GSC_HIP(hipStreamSynchronize(stream));
-                }
-            } catch (...) {
-                (void)hipFree(dmin);
-                (void)hipFree(dmax);
-                (void)hipFree(dval);
-                throw;
-            }
-            (void)hipFree(dmin);
-            (void)hipFree(dmax);
-            (void)hipFree(dval);
+            const auto tx = std::chrono::steady_clock::now();
+            export_triplet(out);
+            export_ms = ms_since(tx);
         }
         if (out->iteration_stats) {
             const uint32_t n = std::min<uint32_t>(out->iteration_stats_capacity, static_cast<uint32_t>(iter_stats.size()));

@@ -13,6 +13,7 @@
 #include <string>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "kernels.hpp"
@@ -35,6 +36,8 @@ struct ConfigError : std::runtime_error {
 
 constexpr uint32_t kBatch = 8;          // iterations launched per host round trip
 constexpr uint32_t kEventsPerIter = 4;  // start, after prefix, after satisfy, after finish
+constexpr uint64_t kExportSlice = 1ull << 22;         // pairs per export slice (37.7 MB of triplet)
+constexpr uint64_t kExportSliceNoAlloc = 1ull << 20;  // the same without allocation (9.4 MB)
 
 inline double ms_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -128,8 +131,13 @@ class DeviceArena {
         GSC_HIP(hipMalloc(&base_, capacity_));
         free_[0] = capacity_;
     }
+    // Over device memory the caller owns (256-byte aligned): nothing is
+    // allocated or freed.
+    DeviceArena(void* base, uint64_t bytes) : base_(base), capacity_(bytes & ~uint64_t(255)), owns_(false) {
+        free_[0] = capacity_;
+    }
     ~DeviceArena() {
-        if (base_) (void)hipFree(base_);
+        if (base_ && owns_) (void)hipFree(base_);
     }
     DeviceArena(const DeviceArena&) = delete;
     DeviceArena& operator=(const DeviceArena&) = delete;
@@ -194,6 +202,7 @@ class DeviceArena {
     uint64_t capacity_ = 0;
     uint64_t in_use_ = 0;
     uint64_t peak_ = 0;
+    bool owns_ = true;
     std::map<uint64_t, uint64_t> free_;  // offset -> size
     std::map<uint64_t, uint64_t> used_;  // offset -> size
     std::mutex m_;
@@ -214,7 +223,7 @@ struct gsc_solver {
     hipStream_t stream = nullptr;
     bool owns_stream = true;
     DeviceArena* arena = nullptr;  // when set, every device buffer comes from it
-    std::vector<void*> allocations;
+    std::vector<std::pair<void*, uint64_t>> allocations;  // (pointer, bytes) from alloc()
     uint64_t bytes_allocated = 0;
 
     uint8_t *valid = nullptr, *cand = nullptr, *reach = nullptr;
@@ -250,7 +259,32 @@ struct gsc_solver {
     IterRecord* host_records = nullptr; // pinned, allocated by the first solve()
     uint32_t* host_words = nullptr;     // pinned scratch words (overflow count, mask counts)
     bool owns_host_words = true;        // false when a server lends them from its pool
+    bool owns_host_ctl = true;          // false when host_ctl / host_records are lent
     std::vector<hipEvent_t> events;
+
+    // Host side of the model tables, rebuilt by every upload_tables().
+    struct HostTables {
+        std::vector<double2> cs;
+        std::vector<double> vt, rad;
+        std::vector<uint2> box2, box3, hull2, hull3;
+        std::vector<uint32_t> fail;
+    };
+    HostTables tables;
+
+    // Abstraction export staging (export_triplet), allocated by the first export.
+    struct ExportStage {
+        uint32_t *dmin = nullptr, *dmax = nullptr;  // device (from alloc)
+        uint8_t* dval = nullptr;
+        uint32_t *hmin = nullptr, *hmax = nullptr;  // pinned host
+        uint8_t* hval = nullptr;
+        hipEvent_t done = nullptr;
+    };
+    ExportStage xstage[2];
+    uint64_t export_slice = 0;
+    double export_ms = 0.0;
+    // No allocation after set-up by anything but the arena: the null stream, no
+    // HIP events, host words lent by the caller, no pinned export staging.
+    bool no_alloc = false;
 
     uint64_t candidate_count = 0;
     // Satisfaction tile: cells per workgroup and inputs per block of record loads.
@@ -278,21 +312,40 @@ struct gsc_solver {
         } else {
             GSC_HIP(hipMalloc(&p, bytes));
         }
-        allocations.push_back(p);
+        allocations.emplace_back(p, bytes);
         bytes_allocated += bytes;
         return static_cast<T*>(p);
     }
 
+    // Gives one buffer from alloc() back before the solver ends.
+    void release(void* p) {
+        auto it = std::find_if(allocations.begin(), allocations.end(), [p](const auto& a) { return a.first == p; });
+        if (it == allocations.end()) return;
+        bytes_allocated -= it->second;
+        allocations.erase(it);
+        if (stream || no_alloc) GSC_HIP(hipStreamSynchronize(stream));
+        if (arena)
+            arena->release(p);
+        else
+            GSC_HIP(hipFree(p));
+    }
+
     ~gsc_solver() {
-        if (stream) (void)hipStreamSynchronize(stream);
-        for (void* p : allocations) {
+        if (stream || no_alloc) (void)hipStreamSynchronize(stream);
+        for (const auto& a : allocations) {
             if (arena)
-                arena->release(p);
+                arena->release(a.first);
             else
-                (void)hipFree(p);
+                (void)hipFree(a.first);
         }
-        if (host_ctl) (void)hipHostFree(host_ctl);
-        if (host_records) (void)hipHostFree(host_records);
+        for (ExportStage& st : xstage) {
+            if (st.hmin) (void)hipHostFree(st.hmin);
+            if (st.hmax) (void)hipHostFree(st.hmax);
+            if (st.hval) (void)hipHostFree(st.hval);
+            if (st.done) (void)hipEventDestroy(st.done);
+        }
+        if (host_ctl && owns_host_ctl) (void)hipHostFree(host_ctl);
+        if (host_records && owns_host_ctl) (void)hipHostFree(host_records);
         if (host_words && owns_host_words) (void)hipHostFree(host_words);
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
         if (stream && owns_stream) (void)hipStreamDestroy(stream);
@@ -309,12 +362,14 @@ struct gsc_solver {
     void build_prefix(const uint8_t* a, const uint8_t* b, uint32_t* P0, uint32_t* P1, const IterCtl* ctl_or_null);
     // Coarse sample of P0 / P1 into C0 / C1 (by iteration parity with ctl).
     void build_coarse(const uint32_t* P0, const uint32_t* P1, uint32_t* C0, uint32_t* C1, const IterCtl* ctl_or_null);
+    void upload_tables();         // model tables, host and device, enqueued on `stream`
     void build_abstraction();     // launch_abstraction() + finish_abstraction()
     void launch_abstraction();    // enqueue on `stream`, no wait
     void finish_abstraction();    // wait; rerun once with a larger overflow table if needed
     void seed_async();            // fixpoint seeds and control reset, enqueued on `stream`
     void solve();
     void download(const gsc_result_buffers* out, gsc_run_report* report);
+    void export_triplet(const gsc_result_buffers* out);
 };
 
 inline int status_of(const DeviceError& e) {

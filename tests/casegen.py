@@ -63,6 +63,21 @@ def small_random_case(seed: int, max_iterations: int = 60) -> Case:
     return case
 
 
+def overflow_regrowth_case() -> Case:
+    """x wraps (wrap_dim 0) and the x disturbance is wide, so 212,480 of the
+    326,656 valid successor boxes wrap in x and cannot be packed in a record
+    word: more than three times the 65,536 overflow entries a solver starts
+    with, so the abstraction regrows its overflow table and runs again."""
+    lb = (0.0, 0.0, -PI_LITERAL, 0.0)
+    ub = (6.4, 6.4, PI_LITERAL, 1.0)
+    shape = (64, 64, 8, 4)
+    eta = tuple(fit_eta(ub[d] - lb[d], shape[d]) for d in range(4))
+    return Case(name="overflow_regrowth", state_lb=lb, state_ub=ub, state_eta=eta, wrap_dim=0,
+                input_lb=(-1.0, -1.0), input_ub=(1.0, 1.0), input_eta=(1.0, 1.0),
+                candidate=_r((0.0, 1.0, -PI_LITERAL, 0.0), (6.4, 5.4, PI_LITERAL, 1.0)), map=_r(lb, ub),
+                disturbance_half_width=(3.0, 0.05, 0.02, 0.02), sample_time=0.5, max_iterations=40)
+
+
 SOLVE_KEYS = ("valid_mask", "candidate_mask", "reachable_mask", "controller", "reach_step")
 
 
@@ -99,7 +114,8 @@ def assert_same_abstraction(gpu, ref, label=""):
 def uneven_cases(oracle, n_small: int = 8, n_random: int = 4):
     """Cases of uneven grid and input sizes for the batched and served paths:
     valid small random cases (configuration errors skipped), 16..100-input
-    instances with ~9:10:4:1 grids, the 30x30x24x10 case and a case with
+    instances with ~9:10:4:1 grids, the 30x30x24x10 case, a case whose
+    overflow table regrows (overflow_regrowth_case) and a case with
     max_iterations = 0."""
     from gsc_amd import cases as gcases
 
@@ -117,6 +133,7 @@ def uneven_cases(oracle, n_small: int = 8, n_random: int = 4):
     for k in range(n_random):
         out.append(gcases.random_instance(rng, 3e3, 2e4, f"uneven_{k}"))
     out.append(gcases.unicycle_30x30x24x10(sample_time=0.5))
+    out.append(overflow_regrowth_case())
     z = small_random_case(3)
     z.name = "zero_iterations"
     z.max_iterations = 0

@@ -7,7 +7,7 @@ controller and reach_step tables, per-iteration counts, stop message.
 import numpy as np
 import pytest
 
-from casegen import assert_same_abstraction, assert_same_solution, small_random_case
+from casegen import assert_same_abstraction, assert_same_solution, overflow_regrowth_case, small_random_case
 from gsc_amd import cases
 from gsc_amd.solver import GscError, Solver, run_case_u32
 
@@ -154,3 +154,16 @@ def test_max_iterations_zero(gpu, oracle):
     got = run_case_u32(c)
     assert_same_solution(got, ref, "max0")
     assert got.result_message == "maximum iterations reached"
+
+
+def test_overflow_table_regrowth(gpu, oracle):
+    """More escaped boxes than the first overflow table holds: the abstraction
+    frees that table, allocates one of the needed size and runs again; the
+    triplet (escaped boxes included) and the solve stay bit-exact."""
+    case = overflow_regrowth_case()
+    got, ref = _check(case, oracle, abstraction=True, label="overflow regrowth")
+    mn, mx, va = ref["min_flat"], ref["max_flat"], ref["pair_valid"].astype(bool)
+    n0 = 64  # x cells
+    wrapped = va & ((mx % n0) < (mn % n0))
+    assert wrapped.sum() > 65536  # the case really needs the regrowth
+    assert got.memory_stats["abstraction_bytes"] >= case.state_count() * case.input_count() * 4 + wrapped.sum() * 16
