@@ -942,3 +942,117 @@ int gsc_run_case_u32(const gsc_case_desc* cfg, const gsc_result_buffers* out, gs
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// gsc_run_case_u32_static: the same run with every device buffer carved out of
+// one file-scope device array, so the call allocates nothing.
+// ===========================================================================
+namespace {
+
+// Each device that loads this library holds its own copy of the array until
+// the process ends. In host code the name is only a stand-in for the symbol;
+// the device address comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) unsigned char scratch[GSC_STATIC_SCRATCH_BYTES];
+
+constexpr int kMaxStaticDevices = 64;
+
+// Per device: the symbol's address on that device, a lock that serialises the
+// calls sharing its array, and the host words the fixpoint loop reads back
+// (pageable; nothing pinned is allocated).
+struct StaticDevice {
+    std::mutex m;
+    void* base = nullptr;
+    IterCtl ctl{};
+    IterRecord records[kRing]{};
+    uint32_t words[4]{};
+};
+StaticDevice g_static[kMaxStaticDevices];
+
+}  // namespace
+
+extern "C" int gsc_run_case_u32_static(const gsc_case_desc* cfg, const gsc_result_buffers* out,
+                                       gsc_run_report* report) {
+    if (!cfg || !report) return GSC_ERR_INVALID_ARGUMENT;
+    std::memset(report, 0, sizeof(*report));
+    report->compiled_with_gpu = 1;
+    int n = 0, dev = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0 || hipGetDevice(&dev) != hipSuccess) {
+        put(report->message, sizeof(report->message), "no HIP device found");
+        return GSC_ERR_NO_DEVICE;
+    }
+    if (dev < 0 || dev >= kMaxStaticDevices) {
+        put(report->message, sizeof(report->message), "device ordinal beyond the static scratch table");
+        return GSC_ERR_INVALID_ARGUMENT;
+    }
+    report->runtime_available = 1;
+    StaticDevice& sd = g_static[dev];
+    std::lock_guard<std::mutex> lock(sd.m);
+    if (!sd.base) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            put(report->message, sizeof(report->message),
+                std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return GSC_ERR_DEVICE;
+        }
+        sd.base = p;
+    }
+
+    // The size check: every buffer of the case plus the export staging.
+    gsc_case_shape sh{};
+    try {
+        sh = make_shape(*cfg);
+    } catch (const ConfigError& e) {
+        put(report->message, sizeof(report->message), e.what());
+        return GSC_ERR_CONFIG;
+    }
+    const bool exporting = out && (out->abstraction_min_flat || out->abstraction_max_flat || out->abstraction_valid);
+    const uint64_t stage = exporting ? 3 * DeviceArena::round_up(std::min<uint64_t>(sh.pair_count, kExportSliceNoAlloc) * 4)
+                                     : 0;
+    const uint64_t need = gsc_solver::device_bytes(sh, *cfg) + stage;
+    if (need > GSC_STATIC_SCRATCH_BYTES) {
+        char msg[GSC_MESSAGE_LEN];
+        std::snprintf(msg, sizeof(msg),
+                      "case needs %llu bytes of device memory; gsc_run_case_u32_static holds at most %llu "
+                      "(GSC_STATIC_SCRATCH_BYTES)",
+                      static_cast<unsigned long long>(need), static_cast<unsigned long long>(GSC_STATIC_SCRATCH_BYTES));
+        put(report->message, sizeof(report->message), msg);
+        return GSC_ERR_OUT_OF_MEMORY;
+    }
+
+    DeviceArena arena(sd.base, GSC_STATIC_SCRATCH_BYTES);
+    gsc_solver s;
+    gsc_solver_default_options(&s.opts);
+    s.opts.device = dev;
+    s.opts.time_kernels = 0;
+    s.opts.verbose = cfg->verbose;
+    copy_case(&s, *cfg);
+    s.no_alloc = true;
+    s.arena = &arena;
+    s.host_words = sd.words;
+    s.owns_host_words = false;
+    s.host_ctl = &sd.ctl;
+    s.host_records = sd.records;
+    s.owns_host_ctl = false;
+    int rc = GSC_OK;
+    try {
+        s.setup();
+        s.build_abstraction();
+        s.solve();
+        s.download(out, report);
+    } catch (const ConfigError& e) {
+        s.last_error = e.what();
+        rc = GSC_ERR_CONFIG;
+    } catch (const DeviceError& e) {
+        s.last_error = e.what();
+        rc = status_of(e);
+    } catch (const std::exception& e) {
+        s.last_error = e.what();
+        rc = GSC_ERR_DEVICE;
+    }
+    if (rc != GSC_OK) {
+        put(report->message, sizeof(report->message), s.last_error);
+        report->executed = 0;
+    }
+    return rc;
+}

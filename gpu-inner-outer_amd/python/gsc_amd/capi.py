@@ -310,6 +310,7 @@ SYMBOLS = [
     ("gsc_device_count", c_int, []),
     ("gsc_case_shape_of", c_int, [POINTER(CaseDesc), POINTER(CaseShape), c_char_p, c_size_t]),
     ("gsc_run_case_u32", c_int, [POINTER(CaseDesc), POINTER(ResultBuffers), POINTER(RunReport)]),
+    ("gsc_run_case_u32_static", c_int, [POINTER(CaseDesc), POINTER(ResultBuffers), POINTER(RunReport)]),
     ("gsc_solver_default_options", None, [POINTER(SolverOptions)]),
     ("gsc_solver_create", c_int, [POINTER(CaseDesc), POINTER(SolverOptions), POINTER(c_void_p), c_char_p, c_size_t]),
     ("gsc_solver_shape", c_int, [c_void_p, POINTER(CaseShape)]),

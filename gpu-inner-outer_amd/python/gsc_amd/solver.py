@@ -145,6 +145,21 @@ def run_case_u32(case: Case, abstraction: bool = False) -> RunResult:
     return _result_from(rep, arrs, stats)
 
 
+def run_case_u32_static(case: Case, abstraction: bool = False) -> RunResult:
+    """gsc_run_case_u32_static — run_case_u32 with every device buffer taken from
+    the library's fixed device array (no allocation). Raises GscError for
+    configuration errors and for cases larger than the array."""
+    lib = capi.load()
+    sh = shape_of(case)
+    d, keep = case.to_desc()
+    b, arrs, stats = _make_buffers(sh["state_count"], sh["pair_count"], case.max_iterations, abstraction)
+    rep = capi.RunReport()
+    rc = lib.gsc_run_case_u32_static(ctypes.byref(d), ctypes.byref(b), ctypes.byref(rep))
+    if rc in (capi.GSC_ERR_CONFIG, capi.GSC_ERR_INVALID_ARGUMENT, capi.GSC_ERR_OUT_OF_MEMORY):
+        raise GscError(rc, rep.message.decode(errors="replace"))
+    return _result_from(rep, arrs, stats)
+
+
 class Solver:
     """One case resident on one device (gsc_solver_* entry points)."""
 

@@ -244,6 +244,24 @@ int gsc_case_shape_of(const gsc_case_desc* cfg, gsc_case_shape* out, char* err, 
  */
 int gsc_run_case_u32(const gsc_case_desc* cfg, const gsc_result_buffers* out, gsc_run_report* report);
 
+/*
+ * gsc_run_case_u32 that allocates nothing. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (serving loops and
+ * harnesses that forbid hipMalloc, code that must not allocate while other
+ * work is captured or in flight). Every device buffer of the run comes from one
+ * file-scope device array of GSC_STATIC_SCRATCH_BYTES per device, held from
+ * library load until the process ends; no hipMalloc, hipFree, pinned-host
+ * allocation, stream or event creation happens in the call. It runs on the
+ * calling thread's current device, on the null stream, and returns the same
+ * results as gsc_run_case_u32. A case whose buffers (plus 12 MiB of export
+ * staging when the abstraction triplet is requested) exceed the array fails
+ * with GSC_ERR_OUT_OF_MEMORY and a message giving both sizes. Calls on one
+ * device share its array (each call re-initialises what it reads) and are
+ * serialised by a lock inside the library: without it they would race.
+ */
+#define GSC_STATIC_SCRATCH_BYTES ((size_t)128 << 20)
+int gsc_run_case_u32_static(const gsc_case_desc* cfg, const gsc_result_buffers* out, gsc_run_report* report);
+
 /* ---------------------------------------------------------------------------
  * Device-resident solver. No reference counterpart: the reference allocates and
  * frees everything per call (kernels.cu:513-560, 860-873). These entry points
