@@ -280,6 +280,27 @@ def batching_section(count=32, seed=3, steps=3):
     }
 
 
+def batch_direct_section(count=32, seed=3, steps=2):
+    """The same 32 cases with the stage kernels launched directly instead of
+    replayed from a captured graph (tile 256 x 8 inputs, 32 iterations per
+    round, largest case first): the form profiled kernel by kernel."""
+    from gsc_amd import cases
+    from gsc_amd.batch import Batch
+
+    cs = cases.batch_instances(count, seed=seed)
+    with Batch(cs, tile_wg=256, tile_ib=8, iterations_per_launch=32, order=1, use_graph=0) as b:
+        b.build_abstraction()
+        b.solve()
+        ms = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            b.build_abstraction()
+            summ = b.solve()
+            ms.append((time.perf_counter() - t0) * 1000.0)
+    return {"cases": count, "direct_ms_per_step": sum(ms) / steps, "rounds": summ["replays"],
+            "max_iterations": summ["max_iterations"]}
+
+
 def serving_section(count=48, seed=5, rates=(100.0, 400.0, 1600.0)):
     """Poisson streams of solve requests (cells log-uniform in [2e4, 4e6]) at
     several rates: the continuous-batching server versus the plain per-call path
@@ -287,7 +308,15 @@ def serving_section(count=48, seed=5, rates=(100.0, 400.0, 1600.0)):
     from gsc_amd import cases
     from gsc_amd.serve import Server, poisson_load, sequential_load
 
+    from gsc_amd.solver import run_case_u32
+
     reqs = cases.serving_requests(count, seed=seed)
+    # Both paths warmed up before any timed stream: the server's kernels and the
+    # per-call path's allocations of the largest requests.
+    for c in sorted(reqs, key=lambda c: -c.state_count() * c.input_count())[:3]:
+        run_case_u32(c)
+    with Server(max_active=4, iterations_per_step=4, hbm_budget_bytes=8 << 30) as srv:
+        poisson_load(srv, reqs[:4], 1e6, seed=0)
     out = []
     identical = None
     for i, rate in enumerate(rates):
@@ -473,6 +502,8 @@ def main():
                     help="measure the dominant kernel's HBM bytes with rocprofv3 --pmc passes")
     ap.add_argument("--pmc-dir", default=os.path.join(ROOT, "gpurun_out", "bench_pmc"))
     ap.add_argument("--copy-calibration", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--batch-direct-only", action="store_true",
+                    help="only the 32-case batch with directly launched stage kernels (for kernel profiles)")
     ap.add_argument("--no-batching", action="store_true", help="skip the 32-case batching comparison")
     ap.add_argument("--no-serving", action="store_true", help="skip the Poisson serving comparison")
     ap.add_argument("--no-extra", action="store_true",
@@ -480,6 +511,9 @@ def main():
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.batch_direct_only:
+        print(json.dumps(batch_direct_section()))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))

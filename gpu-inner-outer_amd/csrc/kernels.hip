@@ -26,7 +26,6 @@ namespace gsc_amd {
 namespace {
 
 __device__ inline uint32_t lane_id() { return threadIdx.x & 63u; }
-
 // Number of set bits of the wave mask m below this lane (mbcnt: two
 // instructions, against a 64-bit lane mask, an and and two popcounts).
 __device__ inline uint32_t lanes_below(uint64_t m) {

@@ -35,6 +35,7 @@ struct Request {
     uint64_t est_bytes = 0;
     Clock::time_point t_submit{}, t_admit{}, t_stop{}, t_done{};
     int slot = -1;
+    bool in_retire = false;  // handed to the retire thread: only it touches the solver now
     uint32_t seen = 0;  // iterations already copied into `stats`
     // Results, on the host once the case stopped.
     std::vector<uint8_t> valid, cand, reach;
@@ -55,7 +56,7 @@ struct gsc_server {
     uint32_t* d_starts = nullptr;
     BatchStatus* d_status = nullptr;
     IterRecord* d_records = nullptr;  // max_active x kRing, by slot
-    uint32_t* h_words = nullptr;      // pinned, 4 words per slot, lent to the slot's solver
+    uint32_t* h_words = nullptr;      // pinned, kHostWords words per slot, lent to the slot's solver
     BatchStatus* h_status = nullptr;  // pinned
     IterRecord* h_records = nullptr;  // pinned
     uint32_t grid[kStages] = {};
@@ -69,13 +70,22 @@ struct gsc_server {
     std::deque<Request*> queue;
     std::unordered_map<uint64_t, std::unique_ptr<Request>> requests;
     std::vector<Request*> active;     // iterating, in grid order (worker only)
-    std::vector<int> free_slots;      // worker only
+    std::vector<int> free_slots;      // guarded by `m`
     bool plan_dirty = true;           // worker only
+    bool trace = false;               // GSC_SERVE_TRACE set: per-step host timings on stderr
     uint64_t next_id = 1;
     bool stopping = false;
     std::string fatal;
     gsc_server_stats stats{};
     std::thread worker;
+    // Stopped cases are copied back by a second thread on its own stream, so
+    // the worker's next step runs while their results travel to the host.
+    hipStream_t retire_stream = nullptr;
+    std::deque<Request*> retire_q;  // guarded by `m`
+    uint32_t retiring = 0;          // handed over, not yet done; guarded by `m`
+    bool retire_stop = false;       // guarded by `m`
+    std::condition_variable cv_retire;
+    std::thread retirer;
 
     ~gsc_server();
     void run();
@@ -84,6 +94,8 @@ struct gsc_server {
     void enqueue_step();
     void admit(std::vector<Request*>& admitted, std::vector<Request*>& prepared);
     void retire(Request* r, const BatchStatus& st);
+    void retire_loop();
+    void finish_retire(Request* r);
     void fail(Request* r, int status, const std::string& why);
     void release_slot(Request* r) {  // caller holds `m`
         if (r->slot >= 0) free_slots.push_back(r->slot);
@@ -107,7 +119,16 @@ gsc_server::~gsc_server() {
     }
     cv_work.notify_all();
     if (worker.joinable()) worker.join();
+    {
+        // The worker has handed over every stopped case; the retire thread
+        // finishes them and leaves.
+        std::lock_guard<std::mutex> lock(m);
+        retire_stop = true;
+    }
+    cv_retire.notify_all();
+    if (retirer.joinable()) retirer.join();
     requests.clear();  // solvers release their arena pieces before the arena goes
+    if (retire_stream) (void)hipStreamSynchronize(retire_stream);
     if (iter_stream) (void)hipStreamSynchronize(iter_stream);
     if (prep_stream) (void)hipStreamSynchronize(prep_stream);
     for (void* p : {static_cast<void*>(d_inst), static_cast<void*>(d_starts), static_cast<void*>(d_status),
@@ -119,6 +140,7 @@ gsc_server::~gsc_server() {
     arena.reset();
     if (iter_stream) (void)hipStreamDestroy(iter_stream);
     if (prep_stream) (void)hipStreamDestroy(prep_stream);
+    if (retire_stream) (void)hipStreamDestroy(retire_stream);
 }
 
 // Caller holds `m`.
@@ -136,7 +158,8 @@ void gsc_server::fail_all(const std::string& why) {
     std::lock_guard<std::mutex> lock(m);
     fatal = why;
     for (auto& kv : requests)
-        if (kv.second->state == GSC_REQUEST_QUEUED || kv.second->state == GSC_REQUEST_RUNNING)
+        if ((kv.second->state == GSC_REQUEST_QUEUED || kv.second->state == GSC_REQUEST_RUNNING) &&
+            !kv.second->in_retire)  // the retire thread finishes those
             fail(kv.second.get(), GSC_ERR_DEVICE, why);
     queue.clear();
     active.clear();
@@ -274,10 +297,12 @@ void gsc_server::admit(std::vector<Request*>& admitted, std::vector<Request*>& p
         s.arena = arena.get();
         s.stream = prep_stream;
         s.owns_stream = false;
-        s.host_words = h_words + 4 * r->slot;
+        s.host_words = h_words + kHostWords * r->slot;
         s.owns_host_words = false;
         try {
-            s.setup();
+            // Masks, valid prefix and abstraction of every admitted case are
+            // enqueued before the first wait on the preparation stream.
+            s.setup_launch();
             s.launch_abstraction();
             launched.push_back(r);
         } catch (const ConfigError& e) {
@@ -303,6 +328,7 @@ void gsc_server::admit(std::vector<Request*>& admitted, std::vector<Request*>& p
             // May regrow the overflow table from the arena (more escaped boxes
             // than the first table held); admission counts the arena's bytes in
             // use, so the regrown table is charged like every other buffer.
+            s.setup_check();
             s.finish_abstraction();
             s.seed_async();
             {
@@ -357,6 +383,36 @@ void gsc_server::retire(Request* r, const BatchStatus& st) {
     s.timings = gsc_kernel_timings{};
     s.timings.abstraction_kernel_ms = s.abstraction_kernel_ms;
     s.solved = true;
+    std::lock_guard<std::mutex> lock(m);
+    r->in_retire = true;
+    ++retiring;
+    retire_q.push_back(r);
+    cv_retire.notify_one();
+}
+
+void gsc_server::retire_loop() {
+    try {
+        GSC_HIP(hipSetDevice(opts.device));
+    } catch (const std::exception&) {
+    }
+    for (;;) {
+        Request* r = nullptr;
+        {
+            std::unique_lock<std::mutex> lock(m);
+            cv_retire.wait(lock, [&] { return retire_stop || !retire_q.empty(); });
+            if (retire_q.empty()) return;
+            r = retire_q.front();
+            retire_q.pop_front();
+        }
+        finish_retire(r);
+    }
+}
+
+// Retire thread: the stopped case's results to host memory on the retire
+// stream (the worker synchronised the step that stopped it), then its HBM back
+// to the arena and the request done.
+void gsc_server::finish_retire(Request* r) {
+    gsc_solver& s = *r->solver;
     const uint64_t S = r->shape.state_count;
     r->valid.resize(S);
     r->cand.resize(S);
@@ -370,14 +426,27 @@ void gsc_server::retire(Request* r, const BatchStatus& st) {
     out.controller = r->ctrl.data();
     out.reach_step = r->rstep.data();
     std::memset(&r->report, 0, sizeof(r->report));
-    s.download(&out, &r->report);
+    std::string error;
+    try {
+        s.stream = retire_stream;
+        s.download(&out, &r->report);
+    } catch (const std::exception& e) {
+        error = e.what();
+    }
     r->solver.reset();  // arena pieces released
     std::lock_guard<std::mutex> lock(m);
     release_slot(r);
-    r->state = GSC_REQUEST_DONE;
-    r->t_done = Clock::now();
-    ++stats.completed;
-    cv_done.notify_all();
+    r->in_retire = false;
+    --retiring;
+    if (!error.empty()) {
+        fail(r, GSC_ERR_DEVICE, error);
+    } else {
+        r->state = GSC_REQUEST_DONE;
+        r->t_done = Clock::now();
+        ++stats.completed;
+        cv_done.notify_all();
+    }
+    cv_work.notify_one();  // HBM and a slot came back
 }
 
 void gsc_server::step_loop() {
@@ -414,6 +483,13 @@ void gsc_server::step_loop() {
                 admitted.push_back(r);
             }
             if (active.empty() && admitted.empty() && !queue.empty()) {
+                if (retiring > 0) {
+                    // Stopped cases still hold HBM and slots until their results
+                    // are on the host: wait for one of them.
+                    const uint32_t before = retiring;
+                    cv_work.wait(lock, [&] { return stopping || retiring < before; });
+                    continue;
+                }
                 // Nothing runs and the head still does not fit: it never will.
                 Request* r = queue.front();
                 queue.pop_front();
@@ -422,17 +498,24 @@ void gsc_server::step_loop() {
             }
         }
         // 2. One step of the admitted cases on the iteration stream...
+        const auto t_step = Clock::now();
         const bool stepping = !active.empty();
         if (stepping) {
             if (plan_dirty) plan();
             enqueue_step();
         }
+        const auto t_enqueued = Clock::now();
         // 3. ...while the newly admitted ones are prepared on the other stream.
         std::vector<Request*> prepared;
         if (!admitted.empty()) admit(admitted, prepared);
+        const auto t_admitted = Clock::now();
         // 4. Collect the step: iteration history, stopped cases leave.
         if (stepping) {
             GSC_HIP(hipStreamSynchronize(iter_stream));
+            if (trace)
+                std::fprintf(stderr, "[serve] step %llu: %zu active, plan+enqueue %.3f ms, %zu admitted %.3f ms, wait %.3f ms\n",
+                             static_cast<unsigned long long>(stats.steps), active.size(), ms_between(t_step, t_enqueued),
+                             admitted.size(), ms_between(t_enqueued, t_admitted), ms_between(t_admitted, Clock::now()));
             ++stats.steps;
             std::vector<Request*> still;
             for (size_t p = 0; p < active.size(); ++p) {
@@ -526,10 +609,12 @@ int gsc_server_create(const gsc_server_options* opts, gsc_server** out, char* er
     }
     auto* s = new gsc_server();
     s->opts = o;
+    s->trace = std::getenv("GSC_SERVE_TRACE") != nullptr;
     try {
         GSC_HIP(hipSetDevice(o.device));
         GSC_HIP(hipStreamCreate(&s->iter_stream));
         GSC_HIP(hipStreamCreate(&s->prep_stream));
+        GSC_HIP(hipStreamCreate(&s->retire_stream));
         const uint32_t A = static_cast<uint32_t>(o.max_active);
         GSC_HIP(hipMalloc(reinterpret_cast<void**>(&s->d_inst), sizeof(BatchInstance) * A));
         GSC_HIP(hipMalloc(reinterpret_cast<void**>(&s->d_starts), sizeof(uint32_t) * kStages * (A + 1)));
@@ -538,13 +623,15 @@ int gsc_server_create(const gsc_server_options* opts, gsc_server** out, char* er
         GSC_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->h_status), sizeof(BatchStatus) * A, hipHostMallocDefault));
         GSC_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->h_records), sizeof(IterRecord) * kRing * A,
                               hipHostMallocDefault));
-        GSC_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->h_words), 4 * sizeof(uint32_t) * A, hipHostMallocDefault));
-        std::memset(s->h_words, 0, 4 * sizeof(uint32_t) * A);
+        GSC_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->h_words), kHostWords * sizeof(uint32_t) * A,
+                              hipHostMallocDefault));
+        std::memset(s->h_words, 0, kHostWords * sizeof(uint32_t) * A);
         std::memset(s->h_status, 0, sizeof(BatchStatus) * A);
         std::memset(s->h_records, 0, sizeof(IterRecord) * kRing * A);
         s->arena = std::make_unique<DeviceArena>(o.device, o.hbm_budget_bytes);
         for (int k = static_cast<int>(A) - 1; k >= 0; --k) s->free_slots.push_back(k);
         s->stats.budget_bytes = s->arena->capacity();
+        s->retirer = std::thread([s] { s->retire_loop(); });
         s->worker = std::thread([s] { s->run(); });
     } catch (const std::exception& e) {
         put(err, err_len, e.what());

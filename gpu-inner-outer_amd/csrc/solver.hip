@@ -135,7 +135,9 @@ void gsc_solver::build_tables(std::vector<double2>& h_cs, std::vector<double>& h
     }
 }
 
-void gsc_solver::setup() {
+// Shape, device buffers, masks and the valid prefix, enqueued on `stream`; the
+// candidate-inside-valid check (abstraction.hpp:261-265) waits in setup_check.
+void gsc_solver::setup_launch() {
     if (cfg.wrap_dim < 0 || cfg.wrap_dim > 3) throw ConfigError("wrap_dim must be 0, 1, 2 or 3");
     shape = make_shape(cfg);
     if (shape.prefix_count > 0xFFFFFFFFull) throw ConfigError("state grid too large for the 32-bit prefix layout");
@@ -240,10 +242,10 @@ void gsc_solver::setup() {
     ctl = alloc<IterCtl>(1);
     records = alloc<IterRecord>(kRing);
     if (!host_words) {
-        GSC_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_words), 4 * sizeof(uint32_t), hipHostMallocDefault));
+        GSC_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_words), kHostWords * sizeof(uint32_t), hipHostMallocDefault));
         owns_host_words = true;
     }
-    std::memset(host_words, 0, 4 * sizeof(uint32_t));
+    std::memset(host_words, 0, kHostWords * sizeof(uint32_t));
     if (!no_alloc) {
         events.resize(opts.time_kernels ? kBatch * kEventsPerIter + 2 : 2);
         for (auto& e : events) GSC_HIP(hipEventCreate(&e));
@@ -300,16 +302,25 @@ void gsc_solver::setup() {
     hipLaunchKernelGGL(masks_kernel, dim3(blocks_for(S, 256)), dim3(256), 0, stream, dc, d_obstacles, d_targets, d_allow, valid,
                        cand, mask_counts);
     GSC_HIP(hipGetLastError());
-    unsigned long long* counts = reinterpret_cast<unsigned long long*>(host_words);
+    unsigned long long* counts = reinterpret_cast<unsigned long long*>(host_words + kWordMaskCounts);
     GSC_HIP(hipMemcpyAsync(counts, mask_counts, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
-    GSC_HIP(hipStreamSynchronize(stream));
-    setup_h2d_ms = ms_since(t0);
-    if (counts[1] != 0) throw ConfigError("candidate set contains invalid states");
-    candidate_count = counts[0];
-
+    // The valid prefix does not wait for the check: a failed check discards it.
     build_prefix(valid, nullptr, prefix_valid, prefix_valid, nullptr);
     build_coarse(prefix_valid, prefix_valid, coarse_valid, coarse_valid, nullptr);
+    setup_t0 = t0;
+}
 
+void gsc_solver::setup_check() {
+    GSC_HIP(hipStreamSynchronize(stream));
+    setup_h2d_ms = ms_since(setup_t0);
+    const unsigned long long* counts = reinterpret_cast<const unsigned long long*>(host_words + kWordMaskCounts);
+    if (counts[1] != 0) throw ConfigError("candidate set contains invalid states");
+    candidate_count = counts[0];
+}
+
+void gsc_solver::setup() {
+    setup_launch();
+    setup_check();
 }
 
 uint64_t gsc_solver::device_bytes(const gsc_case_shape& sh, const gsc_case_desc& cfg) {
@@ -446,12 +457,12 @@ void gsc_solver::launch_abstraction() {
     hipLaunchKernelGGL(abstraction_kernel, dim3(blocks_for(dc.n_tasks, 4)), dim3(256), 0, stream, A);
     GSC_HIP(hipGetLastError());
     if (!events.empty()) GSC_HIP(hipEventRecord(events[1], stream));
-    GSC_HIP(hipMemcpyAsync(host_words, overflow_count, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    GSC_HIP(hipMemcpyAsync(host_words + kWordOverflow, overflow_count, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
 }
 
 void gsc_solver::finish_abstraction() {
     GSC_HIP(hipStreamSynchronize(stream));
-    uint32_t used = host_words[0];
+    uint32_t used = host_words[kWordOverflow];
     if (used > overflow_capacity) {
         // More escaped boxes than the table holds: grow it and run once more.
         if (used >= kRecIndexMask) throw ConfigError("too many irregular successor boxes");
@@ -464,7 +475,7 @@ void gsc_solver::finish_abstraction() {
         launch_abstraction();
         abstraction_t0 = t0;
         GSC_HIP(hipStreamSynchronize(stream));
-        used = host_words[0];
+        used = host_words[kWordOverflow];
         if (used > overflow_capacity) throw ConfigError("successor-box overflow table did not converge");
     }
     abstraction_wall_ms = ms_since(abstraction_t0);
@@ -964,7 +975,7 @@ struct StaticDevice {
     void* base = nullptr;
     IterCtl ctl{};
     IterRecord records[kRing]{};
-    uint32_t words[4]{};
+    uint32_t words[kHostWords]{};
 };
 StaticDevice g_static[kMaxStaticDevices];
 

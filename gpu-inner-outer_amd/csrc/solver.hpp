@@ -36,6 +36,8 @@ struct ConfigError : std::runtime_error {
 
 constexpr uint32_t kBatch = 8;          // iterations launched per host round trip
 constexpr uint32_t kEventsPerIter = 4;  // start, after prefix, after satisfy, after finish
+// Host words a solver reads back: two u64 mask counts, then the overflow count.
+constexpr uint32_t kHostWords = 8, kWordMaskCounts = 0, kWordOverflow = 4;
 constexpr uint64_t kExportSlice = 1ull << 22;         // pairs per export slice (37.7 MB of triplet)
 constexpr uint64_t kExportSliceNoAlloc = 1ull << 20;  // the same without allocation (9.4 MB)
 
@@ -297,6 +299,7 @@ struct gsc_solver {
     double abstraction_kernel_ms = 0.0;
     double abstraction_wall_ms = 0.0;
     double setup_h2d_ms = 0.0;
+    std::chrono::steady_clock::time_point setup_t0{};
     gsc_solve_summary summary{};
     std::vector<gsc_iteration_stats> iter_stats;
     gsc_kernel_timings timings{};
@@ -357,7 +360,9 @@ struct gsc_solver {
 
     void build_tables(std::vector<double2>& h_cs, std::vector<double>& h_vt, std::vector<uint2>& h_box2,
                       std::vector<uint2>& h_box3, std::vector<double>& h_rad, std::vector<uint32_t>& h_fail);
-    void setup();
+    void setup();         // setup_launch() + setup_check()
+    void setup_launch();  // buffers, masks, valid prefix: enqueued on `stream`
+    void setup_check();   // waits, then checks candidate inside valid
     // Table of a & !b; with ctl, into P[k & 1] for the coming iteration k.
     void build_prefix(const uint8_t* a, const uint8_t* b, uint32_t* P0, uint32_t* P1, const IterCtl* ctl_or_null);
     // Coarse sample of P0 / P1 into C0 / C1 (by iteration parity with ctl).
