@@ -37,10 +37,23 @@ struct Request {
     int slot = -1;
     bool in_retire = false;  // handed to the retire thread: only it touches the solver now
     uint32_t seen = 0;  // iterations already copied into `stats`
-    // Results, on the host once the case stopped.
-    std::vector<uint8_t> valid, cand, reach;
-    std::vector<uint32_t> ctrl, rstep;
+    // Results, on the host once the case stopped: one block of 11 B per state
+    // (controller, reach step, then the valid, candidate and reachable masks),
+    // in the server's pinned result pool or, when the pool is full, in
+    // `fallback`. The pool block goes back when the request is destroyed.
+    uint8_t* results = nullptr;
+    DeviceArena* pool = nullptr;  // owner of `results` when it is a pool block
+    std::vector<uint8_t> fallback;
     std::vector<gsc_iteration_stats> stats;
+
+    ~Request() {
+        if (results && pool) pool->release(results);
+    }
+    const uint32_t* ctrl() const { return reinterpret_cast<const uint32_t*>(results); }
+    const uint32_t* rstep() const { return reinterpret_cast<const uint32_t*>(results) + shape.state_count; }
+    const uint8_t* valid() const { return results + 8 * shape.state_count; }
+    const uint8_t* cand() const { return results + 9 * shape.state_count; }
+    const uint8_t* reach() const { return results + 10 * shape.state_count; }
     gsc_solve_summary summary{};
     gsc_run_report report{};
 };
@@ -51,6 +64,10 @@ struct Request {
 struct gsc_server {
     gsc_server_options opts{};
     std::unique_ptr<DeviceArena> arena;
+    // Finished results until fetched: one pinned host block carved like the
+    // device arena (its bookkeeping is plain offset arithmetic).
+    void* pinned_results = nullptr;
+    std::unique_ptr<DeviceArena> result_pool;
     hipStream_t iter_stream = nullptr, prep_stream = nullptr;
     BatchInstance* d_inst = nullptr;
     uint32_t* d_starts = nullptr;
@@ -73,6 +90,7 @@ struct gsc_server {
     std::vector<int> free_slots;      // guarded by `m`
     bool plan_dirty = true;           // worker only
     bool trace = false;               // GSC_SERVE_TRACE set: per-step host timings on stderr
+    Clock::time_point t_created = Clock::now();
     uint64_t next_id = 1;
     bool stopping = false;
     std::string fatal;
@@ -127,7 +145,9 @@ gsc_server::~gsc_server() {
     }
     cv_retire.notify_all();
     if (retirer.joinable()) retirer.join();
-    requests.clear();  // solvers release their arena pieces before the arena goes
+    requests.clear();  // solvers and results release their pieces before the pools go
+    result_pool.reset();
+    if (pinned_results) (void)hipHostFree(pinned_results);
     if (retire_stream) (void)hipStreamSynchronize(retire_stream);
     if (iter_stream) (void)hipStreamSynchronize(iter_stream);
     if (prep_stream) (void)hipStreamSynchronize(prep_stream);
@@ -414,25 +434,35 @@ void gsc_server::retire_loop() {
 void gsc_server::finish_retire(Request* r) {
     gsc_solver& s = *r->solver;
     const uint64_t S = r->shape.state_count;
-    r->valid.resize(S);
-    r->cand.resize(S);
-    r->reach.resize(S);
-    r->ctrl.resize(S);
-    r->rstep.resize(S);
+    // Pinned pool block when one is free (the copies then run at the link's
+    // speed and nothing is zero-filled), pageable memory otherwise.
+    uint8_t* block = result_pool ? static_cast<uint8_t*>(result_pool->allocate(11 * S)) : nullptr;
+    if (block) {
+        r->pool = result_pool.get();
+    } else {
+        r->fallback.resize(11 * S);
+        block = r->fallback.data();
+    }
+    r->results = block;
     gsc_result_buffers out{};
-    out.valid_mask = r->valid.data();
-    out.candidate_mask = r->cand.data();
-    out.reachable_mask = r->reach.data();
-    out.controller = r->ctrl.data();
-    out.reach_step = r->rstep.data();
+    out.controller = reinterpret_cast<uint32_t*>(block);
+    out.reach_step = reinterpret_cast<uint32_t*>(block) + S;
+    out.valid_mask = block + 8 * S;
+    out.candidate_mask = block + 9 * S;
+    out.reachable_mask = block + 10 * S;
     std::memset(&r->report, 0, sizeof(r->report));
     std::string error;
+    const auto t0 = Clock::now();
     try {
         s.stream = retire_stream;
         s.download(&out, &r->report);
     } catch (const std::exception& e) {
         error = e.what();
     }
+    if (trace)
+        std::fprintf(stderr, "[serve] t=%.3f retire request %llu: %llu states copied in %.3f ms\n",
+                     ms_between(t_created, t0), static_cast<unsigned long long>(r->id), static_cast<unsigned long long>(S),
+                     ms_between(t0, Clock::now()));
     r->solver.reset();  // arena pieces released
     std::lock_guard<std::mutex> lock(m);
     release_slot(r);
@@ -513,8 +543,9 @@ void gsc_server::step_loop() {
         if (stepping) {
             GSC_HIP(hipStreamSynchronize(iter_stream));
             if (trace)
-                std::fprintf(stderr, "[serve] step %llu: %zu active, plan+enqueue %.3f ms, %zu admitted %.3f ms, wait %.3f ms\n",
-                             static_cast<unsigned long long>(stats.steps), active.size(), ms_between(t_step, t_enqueued),
+                std::fprintf(stderr, "[serve] t=%.3f step %llu: %zu active, plan+enqueue %.3f ms, %zu admitted %.3f ms, wait %.3f ms\n",
+                             ms_between(t_created, t_step), static_cast<unsigned long long>(stats.steps), active.size(),
+                             ms_between(t_step, t_enqueued),
                              admitted.size(), ms_between(t_enqueued, t_admitted), ms_between(t_admitted, Clock::now()));
             ++stats.steps;
             std::vector<Request*> still;
@@ -589,6 +620,7 @@ void gsc_server_default_options(gsc_server_options* o) {
     o->iterations_per_step = 4;
     o->tile_wg = 256;
     o->tile_ib = 8;
+    o->result_pool_mib = 1024;
     o->hbm_budget_bytes = 64ull << 30;
 }
 
@@ -603,7 +635,7 @@ int gsc_server_create(const gsc_server_options* opts, gsc_server** out, char* er
     if (o.max_active < 1 || o.max_active > 1024 || o.iterations_per_step < 1 ||
         o.iterations_per_step > static_cast<int32_t>(kRing) ||
         !satisfy_variant_supported(static_cast<uint32_t>(o.tile_wg), static_cast<uint32_t>(o.tile_ib)) ||
-        o.hbm_budget_bytes == 0) {
+        o.hbm_budget_bytes == 0 || o.result_pool_mib < 0) {
         put(err, err_len, "invalid server options");
         return GSC_ERR_INVALID_ARGUMENT;
     }
@@ -629,6 +661,11 @@ int gsc_server_create(const gsc_server_options* opts, gsc_server** out, char* er
         std::memset(s->h_status, 0, sizeof(BatchStatus) * A);
         std::memset(s->h_records, 0, sizeof(IterRecord) * kRing * A);
         s->arena = std::make_unique<DeviceArena>(o.device, o.hbm_budget_bytes);
+        if (o.result_pool_mib > 0) {
+            const uint64_t bytes = static_cast<uint64_t>(o.result_pool_mib) << 20;
+            GSC_HIP(hipHostMalloc(&s->pinned_results, bytes, hipHostMallocDefault));
+            s->result_pool = std::make_unique<DeviceArena>(s->pinned_results, bytes);
+        }
         for (int k = static_cast<int>(A) - 1; k >= 0; --k) s->free_slots.push_back(k);
         s->stats.budget_bytes = s->arena->capacity();
         s->retirer = std::thread([s] { s->retire_loop(); });
@@ -725,12 +762,12 @@ int gsc_server_fetch(gsc_server* s, uint64_t id, const gsc_result_buffers* out, 
         return r->status;
     }
     if (out) {
-        const size_t S = r->valid.size();
-        if (out->valid_mask) std::memcpy(out->valid_mask, r->valid.data(), S);
-        if (out->candidate_mask) std::memcpy(out->candidate_mask, r->cand.data(), S);
-        if (out->reachable_mask) std::memcpy(out->reachable_mask, r->reach.data(), S);
-        if (out->controller) std::memcpy(out->controller, r->ctrl.data(), S * 4);
-        if (out->reach_step) std::memcpy(out->reach_step, r->rstep.data(), S * 4);
+        const size_t S = r->shape.state_count;
+        if (out->valid_mask) std::memcpy(out->valid_mask, r->valid(), S);
+        if (out->candidate_mask) std::memcpy(out->candidate_mask, r->cand(), S);
+        if (out->reachable_mask) std::memcpy(out->reachable_mask, r->reach(), S);
+        if (out->controller) std::memcpy(out->controller, r->ctrl(), S * 4);
+        if (out->reach_step) std::memcpy(out->reach_step, r->rstep(), S * 4);
         if (out->iteration_stats) {
             const size_t n = std::min<size_t>(out->iteration_stats_capacity, r->stats.size());
             std::memcpy(out->iteration_stats, r->stats.data(), n * sizeof(gsc_iteration_stats));

@@ -265,7 +265,8 @@ class ServerOptions(ctypes.Structure):
         ("iterations_per_step", c_int32),
         ("tile_wg", c_int32),
         ("tile_ib", c_int32),
-        ("reserved", c_int32 * 3),
+        ("result_pool_mib", c_int32),
+        ("reserved", c_int32 * 2),
         ("hbm_budget_bytes", c_uint64),
     ]
 

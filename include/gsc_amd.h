@@ -409,7 +409,9 @@ typedef struct gsc_server_options {
     int32_t iterations_per_step;  /* fused iterations between admission checks, 1..64 */
     int32_t tile_wg;              /* satisfaction tile, as gsc_batch_options */
     int32_t tile_ib;
-    int32_t reserved[3];
+    int32_t result_pool_mib;      /* pinned host memory holding finished results until fetched
+                                     (11 B per state); results that do not fit go to pageable memory */
+    int32_t reserved[2];
     uint64_t hbm_budget_bytes;    /* device memory reserved for admitted cases */
 } gsc_server_options;
 
