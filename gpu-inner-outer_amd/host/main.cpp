@@ -1,43 +1,62 @@
 // main.cpp — cuda_symbolic_control_cli drop-in (src/main.cpp:125-218) on the
-// gfx950 backend: same arguments, summary and timing printout, HDF5 output
-// and exit codes (0 ok, 1 error or usage, 2 GPU run unavailable).
+// gfx950 backend: same arguments, modes, summary and timing printout, HDF5
+// output and exit codes (0 ok, 1 error or usage, 2 GPU run unavailable,
+// 3 --compare mismatch).
 //
 //   cuda_symbolic_control_cli <config.cfg> [--cpu|--gpu|--compare] [--output result.h5]
 //                             [--no-abstraction-export]
 //
-// --gpu is the backend this library provides. The reference's --cpu and
-// --compare modes run its own CPU solver, which is not part of this backend;
-// they exit 1 with a message. --no-abstraction-export (an addition) skips the
-// per-pair /abstraction datasets, whose device-to-host copy and write dominate
-// the reference's wall time at 401M pairs.
+// --gpu runs the gfx950 backend. --cpu (the default, as in the reference)
+// runs the host solver of gsc/cpu_solver.hpp; --compare runs both and exits 3
+// unless the five result arrays agree exactly (main.cpp:114-121, 189-211).
+// The host solver's threads: GSC_CPU_THREADS, else OMP_NUM_THREADS, else every
+// hardware thread. --no-abstraction-export (an addition) skips the per-pair
+// /abstraction datasets of --gpu, whose device-to-host copy and write dominate
+// the reference's wall time at 401M pairs; --compare never exports them, as
+// its file has no /abstraction group.
 #include <algorithm>
 #include <iostream>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
+#include "gsc/cpu_solver.hpp"
 #include "gsc/hip_api.hpp"
 #include "gsc/result_io.hpp"
 
 namespace {
 
-void print_summary(const gsc::GpuRunReport& gpu) {
+void print_summary(const gsc::Grid4D<std::uint32_t>& state_grid, const gsc::InputGrid2D& input_grid,
+                   const gsc::MemoryBudget& budget, std::uint64_t pair_count,
+                   const std::vector<std::uint8_t>& candidate_mask, const gsc::SolveResult<std::uint32_t>& r,
+                   double abstraction_ms, const std::string& label) {
     // main.cpp:12-39
     std::uint64_t candidate_states = 0;
-    for (auto v : gpu.result.candidate_mask) candidate_states += (v != 0);
-    const auto& r = gpu.result;
-    std::cout << "\n[gpu]\n";
-    std::cout << "states        : " << gpu.state_grid.total_size << '\n';
-    std::cout << "inputs        : " << gpu.input_grid.total_size << '\n';
-    std::cout << "pairs         : " << gpu.pair_count << '\n';
+    for (auto v : candidate_mask) candidate_states += (v != 0);
+    std::cout << "\n[" << label << "]\n";
+    std::cout << "states        : " << state_grid.total_size << '\n';
+    std::cout << "inputs        : " << input_grid.total_size << '\n';
+    std::cout << "pairs         : " << pair_count << '\n';
     std::cout << "reachable     : " << r.reachable_states << '\n';
     std::cout << "candidate     : " << candidate_states << '\n';
     std::cout << "candidate ctl : " << r.certified_candidate_states << '\n';
     std::cout << "iterations    : " << r.iterations << '\n';
     std::cout << "converged     : " << (r.converged ? "true" : "false") << '\n';
     std::cout << "message       : " << r.message << '\n';
-    std::cout << "abstraction   : " << gpu.abstraction_ms << " ms\n";
+    std::cout << "abstraction   : " << abstraction_ms << " ms\n";
     std::cout << "solve         : " << r.solve_ms << " ms\n";
-    std::cout << "memory total  : " << gsc::format_bytes(gpu.budget.total_bytes) << '\n';
+    std::cout << "memory total  : " << gsc::format_bytes(budget.total_bytes) << '\n';
+}
+
+void print_summary(const gsc::CpuRunReport& cpu, const gsc::SolveResult<std::uint32_t>& r, const std::string& label) {
+    print_summary(cpu.state_grid, cpu.input_grid, cpu.budget, cpu.pair_count, cpu.result.candidate_mask, r,
+                  cpu.abstraction_ms, label);
+}
+
+bool same_result(const gsc::SolveResult<std::uint32_t>& a, const gsc::SolveResult<std::uint32_t>& b) {
+    // main.cpp:114-121
+    return a.valid_mask == b.valid_mask && a.candidate_mask == b.candidate_mask &&
+           a.reachable_mask == b.reachable_mask && a.controller == b.controller && a.reach_step == b.reach_step;
 }
 
 void print_iteration(const gsc::IterationStats& s) {
@@ -120,26 +139,61 @@ int main(int argc, char** argv) {
         const std::string mode_name = mode.substr(2);
         if (output_path.empty()) output_path = gsc::default_result_path(cfg, mode_name);
 
-        if (mode != "--gpu")
-            throw std::runtime_error("mode " + mode + " runs the reference's CPU solver, which is not part of this "
-                                     "backend; use --gpu");
-
-        gsc::RunOptions run_opts;
-        run_opts.export_abstraction = export_abstraction;
-        const auto gpu = gsc::run_case_hip_u32(cfg, ext, run_opts);
-        if (!gpu.executed) {
-            std::cerr << "gpu run unavailable: " << gpu.message << '\n';
-            return 2;
+        if (mode == "--cpu") {
+            const auto cpu = gsc::run_case_cpu_u32(cfg, ext);
+            print_summary(cpu, cpu.result, "cpu");
+            gsc::write_results_hdf5(output_path, cfg, cpu.state_grid, cpu.input_grid, cpu.abstraction_ms,
+                                    {{"cpu", &cpu.result}});
+            std::cout << "result file   : " << output_path << '\n';
+            return 0;
         }
-        print_summary(gpu);
-        print_gpu_detailed_timing(gpu);
-        gsc::write_gpu_results_hdf5(output_path, cfg, gpu);
-        std::cout << "\nresult file   : " << output_path << '\n';
-        std::cout << "  - includes abstraction data ("
-                  << (gpu.pair_count * (2 * sizeof(std::uint32_t) + sizeof(std::uint8_t)) / (1024.0 * 1024.0))
-                  << " MB)\n";
-        std::cout << "  - includes " << gpu.iteration_stats.size() << " iteration statistics\n";
-        return 0;
+
+        if (mode == "--gpu") {
+            gsc::RunOptions run_opts;
+            run_opts.export_abstraction = export_abstraction;
+            const auto gpu = gsc::run_case_hip_u32(cfg, ext, run_opts);
+            if (!gpu.executed) {
+                std::cerr << "gpu run unavailable: " << gpu.message << '\n';
+                return 2;
+            }
+            print_summary(gpu.state_grid, gpu.input_grid, gpu.budget, gpu.pair_count, gpu.result.candidate_mask,
+                          gpu.result, gpu.abstraction_ms, "gpu");
+            print_gpu_detailed_timing(gpu);
+            gsc::write_gpu_results_hdf5(output_path, cfg, gpu);
+            std::cout << "\nresult file   : " << output_path << '\n';
+            std::cout << "  - includes abstraction data ("
+                      << (gpu.pair_count * (2 * sizeof(std::uint32_t) + sizeof(std::uint8_t)) / (1024.0 * 1024.0))
+                      << " MB)\n";
+            std::cout << "  - includes " << gpu.iteration_stats.size() << " iteration statistics\n";
+            return 0;
+        }
+
+        if (mode == "--compare") {
+            const auto cpu = gsc::run_case_cpu_u32(cfg, ext);
+            gsc::RunOptions run_opts;
+            run_opts.export_abstraction = false;
+            const auto gpu = gsc::run_case_hip_u32(cfg, ext, run_opts);
+            if (!gpu.executed) {
+                std::cerr << "gpu compare unavailable: " << gpu.message << '\n';
+                print_summary(cpu, cpu.result, "cpu");
+                gsc::write_results_hdf5(output_path, cfg, cpu.state_grid, cpu.input_grid, cpu.abstraction_ms,
+                                        {{"cpu", &cpu.result}});
+                std::cout << "result file   : " << output_path << '\n';
+                return 2;
+            }
+            // main.cpp:200-211: both summaries from the CPU preparation.
+            print_summary(cpu, cpu.result, "cpu");
+            print_summary(cpu, gpu.result, "gpu");
+            print_gpu_detailed_timing(gpu);
+            gsc::write_results_hdf5(output_path, cfg, cpu.state_grid, cpu.input_grid, cpu.abstraction_ms,
+                                    {{"cpu", &cpu.result}, {"gpu", &gpu.result}});
+            const bool match = same_result(cpu.result, gpu.result);
+            std::cout << "\nresult file   : " << output_path << '\n';
+            std::cout << "exact_match   : " << (match ? "true" : "false") << '\n';
+            return match ? 0 : 3;
+        }
+
+        throw std::runtime_error("unknown mode: " + mode);
     } catch (const std::exception& ex) {
         std::cerr << "error: " << ex.what() << '\n';
         return 1;

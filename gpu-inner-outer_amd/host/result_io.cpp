@@ -1,5 +1,6 @@
-// result_io.cpp — HDF5 output of a GPU run with the reference's layout
-// (src/result_io.cpp:364-517): contiguous, uncompressed, native-endian
+// result_io.cpp — HDF5 result files with the reference's layouts: a GPU run
+// (src/result_io.cpp:364-517) and the --cpu / --compare runs
+// (src/result_io.cpp:238-362). Contiguous, uncompressed, native-endian
 // datasets; fixed-length NULLTERM strings; readers reshape with order="F"
 // (x fastest).
 #include "gsc/result_io.hpp"
@@ -109,60 +110,38 @@ void string(hid_t parent, const char* name, const std::string& value) {
     check(H5Dwrite(ds.get(), type.get(), H5S_ALL, H5S_ALL, H5P_DEFAULT, value.c_str()), std::string("write ") + name);
 }
 
-#endif
-
-}  // namespace
-
-bool hdf5_export_compiled() {
-#if GSC_HAS_HDF5
-    return true;
-#else
-    return false;
-#endif
-}
-
-std::string default_result_path(const CaseConfig& cfg, const std::string& mode) {
-    return (std::filesystem::path("results") / (sanitize(cfg.name) + "_" + sanitize(mode) + ".h5")).string();
-}
-
-void write_gpu_results_hdf5(const std::string& output_path, const CaseConfig& cfg, const GpuRunReport& r) {
-#if GSC_HAS_HDF5
-    if (!r.executed) throw std::runtime_error("GPU report has not been executed");
-    const std::filesystem::path out(output_path);
-    if (out.has_parent_path()) std::filesystem::create_directories(out.parent_path());
-    Id file(H5Fcreate(out.string().c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), &H5Fclose,
-            "cannot create file " + out.string());
-    const hid_t f = file.get();
-
+// /meta, /state, /input and /config (result_io.cpp:238-290, 380-440).
+void write_header(hid_t f, const CaseConfig& cfg, const Grid4D<std::uint32_t>& sg, const InputGrid2D& ig,
+                  std::uint32_t run_count) {
     {
         Id meta = group(f, "meta");
         string(meta.get(), "case_name", cfg.name);
         scalar<std::uint32_t>(meta.get(), "invalid_input", kInvalidInput);
         scalar<std::uint32_t>(meta.get(), "unreachable_step", kUnreachableStep);
-        scalar<std::uint32_t>(meta.get(), "run_count", 1);
+        scalar<std::uint32_t>(meta.get(), "run_count", run_count);
     }
     {
         Id st = group(f, "state");
-        vec(st.get(), "lb", r.state_grid.lb, kStateDim);
-        vec(st.get(), "ub", r.state_grid.ub, kStateDim);
-        vec(st.get(), "eta", r.state_grid.eta, kStateDim);
-        vec(st.get(), "shape", r.state_grid.shape, kStateDim);
-        vec(st.get(), "strides", r.state_grid.strides, kStateDim);
-        scalar<std::uint64_t>(st.get(), "total_size", r.state_grid.total_size);
-        scalar<int>(st.get(), "wrap_dim", r.state_grid.wrap_dim);
+        vec(st.get(), "lb", sg.lb, kStateDim);
+        vec(st.get(), "ub", sg.ub, kStateDim);
+        vec(st.get(), "eta", sg.eta, kStateDim);
+        vec(st.get(), "shape", sg.shape, kStateDim);
+        vec(st.get(), "strides", sg.strides, kStateDim);
+        scalar<std::uint64_t>(st.get(), "total_size", sg.total_size);
+        scalar<int>(st.get(), "wrap_dim", sg.wrap_dim);
     }
     {
         Id in = group(f, "input");
-        vec(in.get(), "lb", r.input_grid.lb, kInputDim);
-        vec(in.get(), "ub", r.input_grid.ub, kInputDim);
-        vec(in.get(), "eta", r.input_grid.eta, kInputDim);
-        vec(in.get(), "shape", r.input_grid.shape, kInputDim);
-        vec(in.get(), "strides", r.input_grid.strides, kInputDim);
-        scalar<std::uint64_t>(in.get(), "total_size", r.input_grid.total_size);
-        std::vector<double> centers(r.input_grid.total_size * kInputDim);
-        for (std::uint64_t k = 0; k < r.input_grid.total_size; ++k)
-            r.input_grid.center(static_cast<InputIndex>(k), &centers[k * kInputDim]);
-        matrix(in.get(), "centers", centers.data(), r.input_grid.total_size, kInputDim);
+        vec(in.get(), "lb", ig.lb, kInputDim);
+        vec(in.get(), "ub", ig.ub, kInputDim);
+        vec(in.get(), "eta", ig.eta, kInputDim);
+        vec(in.get(), "shape", ig.shape, kInputDim);
+        vec(in.get(), "strides", ig.strides, kInputDim);
+        scalar<std::uint64_t>(in.get(), "total_size", ig.total_size);
+        std::vector<double> centers(ig.total_size * kInputDim);
+        for (std::uint64_t k = 0; k < ig.total_size; ++k)
+            ig.center(static_cast<InputIndex>(k), &centers[k * kInputDim]);
+        matrix(in.get(), "centers", centers.data(), ig.total_size, kInputDim);
     }
     {
         Id cf = group(f, "config");
@@ -182,26 +161,90 @@ void write_gpu_results_hdf5(const std::string& output_path, const CaseConfig& cf
         }
         matrix(cf.get(), "obstacles", obst.data(), cfg.obstacles.size(), 2 * kStateDim);
     }
+}
+
+// /summary/<label> and /result/<label> of one run (result_io.cpp:292-315).
+void write_run(hid_t summary, hid_t result, const std::string& label, const SolveResult<std::uint32_t>& r,
+               double abstraction_ms, double solve_ms, const std::uint64_t* pair_count) {
+    const std::string name = sanitize(label);
     {
-        Id summary = group(f, "summary");
-        Id g = group(summary.get(), "gpu");
-        scalar<std::uint32_t>(g.get(), "iterations", r.result.iterations);
-        scalar<std::uint8_t>(g.get(), "converged", r.result.converged ? 1 : 0);
-        string(g.get(), "message", r.result.message);
-        scalar<std::uint64_t>(g.get(), "reachable_states", r.result.reachable_states);
-        scalar<std::uint64_t>(g.get(), "certified_candidate_states", r.result.certified_candidate_states);
-        scalar<double>(g.get(), "abstraction_ms", r.abstraction_ms);
-        scalar<double>(g.get(), "solve_ms", r.solve_ms);
-        scalar<std::uint64_t>(g.get(), "pair_count", r.pair_count);
+        Id g = group(summary, name.c_str());
+        scalar<std::uint32_t>(g.get(), "iterations", r.iterations);
+        scalar<std::uint8_t>(g.get(), "converged", r.converged ? 1 : 0);
+        string(g.get(), "message", r.message);
+        scalar<std::uint64_t>(g.get(), "reachable_states", r.reachable_states);
+        scalar<std::uint64_t>(g.get(), "certified_candidate_states", r.certified_candidate_states);
+        scalar<double>(g.get(), "abstraction_ms", abstraction_ms);
+        scalar<double>(g.get(), "solve_ms", solve_ms);
+        if (pair_count) scalar<std::uint64_t>(g.get(), "pair_count", *pair_count);
     }
     {
+        Id g = group(result, name.c_str());
+        vec(g.get(), "valid_mask", r.valid_mask);
+        vec(g.get(), "candidate_mask", r.candidate_mask);
+        vec(g.get(), "reachable_mask", r.reachable_mask);
+        vec(g.get(), "controller", r.controller);
+        vec(g.get(), "reach_step", r.reach_step);
+    }
+}
+
+Id create_file(const std::string& output_path) {
+    const std::filesystem::path out(output_path);
+    if (out.has_parent_path()) std::filesystem::create_directories(out.parent_path());
+    return Id(H5Fcreate(out.string().c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), &H5Fclose,
+              "cannot create file " + out.string());
+}
+
+#endif
+
+}  // namespace
+
+bool hdf5_export_compiled() {
+#if GSC_HAS_HDF5
+    return true;
+#else
+    return false;
+#endif
+}
+
+std::string default_result_path(const CaseConfig& cfg, const std::string& mode) {
+    return (std::filesystem::path("results") / (sanitize(cfg.name) + "_" + sanitize(mode) + ".h5")).string();
+}
+
+void write_results_hdf5(const std::string& output_path, const CaseConfig& cfg, const Grid4D<std::uint32_t>& state_grid,
+                        const InputGrid2D& input_grid, double abstraction_ms,
+                        const std::vector<ResultExportView>& runs) {
+#if GSC_HAS_HDF5
+    if (runs.empty()) throw std::runtime_error("no runs were provided for HDF5 export");
+    Id file = create_file(output_path);
+    write_header(file.get(), cfg, state_grid, input_grid, static_cast<std::uint32_t>(runs.size()));
+    Id summary = group(file.get(), "summary");
+    Id result = group(file.get(), "result");
+    for (const auto& run : runs) {
+        if (!run.result) throw std::runtime_error("result export view has null result pointer");
+        write_run(summary.get(), result.get(), run.label, *run.result, abstraction_ms, run.result->solve_ms, nullptr);
+    }
+#else
+    (void)output_path;
+    (void)cfg;
+    (void)state_grid;
+    (void)input_grid;
+    (void)abstraction_ms;
+    (void)runs;
+    throw std::runtime_error("HDF5 export support was not compiled");
+#endif
+}
+
+void write_gpu_results_hdf5(const std::string& output_path, const CaseConfig& cfg, const GpuRunReport& r) {
+#if GSC_HAS_HDF5
+    if (!r.executed) throw std::runtime_error("GPU report has not been executed");
+    Id file = create_file(output_path);
+    const hid_t f = file.get();
+    write_header(f, cfg, r.state_grid, r.input_grid, 1);
+    {
+        Id summary = group(f, "summary");
         Id result = group(f, "result");
-        Id g = group(result.get(), "gpu");
-        vec(g.get(), "valid_mask", r.result.valid_mask);
-        vec(g.get(), "candidate_mask", r.result.candidate_mask);
-        vec(g.get(), "reachable_mask", r.result.reachable_mask);
-        vec(g.get(), "controller", r.result.controller);
-        vec(g.get(), "reach_step", r.result.reach_step);
+        write_run(summary.get(), result.get(), "gpu", r.result, r.abstraction_ms, r.solve_ms, &r.pair_count);
     }
     {
         Id a = group(f, "abstraction");
