@@ -220,7 +220,9 @@ void gsc_batch::plan() {
         b.sat.hull3 = s.hull3;
         b.sat.prefix_valid = s.prefix_valid;
         b.sat.unreached_bits = s.ubits;
-        b.sat.probe_zw = s.probe_zw;
+        b.sat.probe_base = s.probe_base;
+        // Written by chunk_flags_batch_kernel, which runs where frontier skipping does.
+        b.sat.probe_or_bits = s.probe_or_used() ? s.probe_or_bits : nullptr;
         b.sat.rec = s.rec;
         b.sat.overflow = s.overflow;
         b.sat.box2 = s.box2;

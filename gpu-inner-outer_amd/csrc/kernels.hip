@@ -556,11 +556,16 @@ __global__ __launch_bounds__(256) void increment_sums_kernel(DevCase c, const do
     }
 }
 
-// θ/v part of the probe cell per (plane, input): lo2 * stride2 + lo3 * stride3 of
-// the pair's successor box (0 for padding inputs and rejected ranges, whose
-// records are invalid and never probed).
-__global__ __launch_bounds__(256) void probe_zw_kernel(DevCase c, const uint2* __restrict__ box2,
-                                                        const uint2* __restrict__ box3, uint32_t* __restrict__ probe_zw) {
+// θ/v part of the probe row per (plane, input): the bit where the plane
+// lo2 + n_θ * lo3 of the pair's successor box starts in the probe bitmasks. A
+// box whose θ range holds lo2 and the next θ cell (the range is longer than one
+// cell, or wraps) and whose v range holds lo3 and the next v cell probes the OR
+// bitmask, which covers those four slices, when it is built (use_or). 0 for
+// padding inputs and rejected ranges, whose records are invalid and never
+// probed.
+__global__ __launch_bounds__(256) void probe_base_kernel(DevCase c, const uint2* __restrict__ box2,
+                                                          const uint2* __restrict__ box3, uint32_t use_or,
+                                                          uint32_t* __restrict__ probe_base) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = c.shape[2] * c.shape[3] * c.I_pad;
     if (t >= n) return;
@@ -571,9 +576,13 @@ __global__ __launch_bounds__(256) void probe_zw_kernel(DevCase c, const uint2* _
         const uint32_t iw = i / c.n_a;
         const uint2 b2 = box2[(plane % c.shape[2]) * c.n_w + iw];
         const uint2 b3 = box3[(plane / c.shape[2]) * c.n_a + (i - iw * c.n_a)];
-        if (b2.x != kBoxReject && b3.x != kBoxReject) v = b2.x * c.stride[2] + b3.x * c.stride[3];
+        if (b2.x != kBoxReject && b3.x != kBoxReject) {
+            const uint32_t pz = b2.x + c.shape[2] * b3.x;
+            const bool span = b2.y != b2.x && b3.y != b3.x;
+            v = use_or && span ? probe_or_bit0(c) + pz * probe_or_pitch(c) : pz * c.PQ;
+        }
     }
-    probe_zw[t] = v;
+    probe_base[t] = v;
 }
 
 // The kernel's own argument block, addressed as ordinary (global) memory, so
@@ -963,8 +972,8 @@ __device__ inline bool count_test(const SatisfyArgs& A, const uint32_t* PU, uint
 // — and, if neither holds one, the 16-corner count (count_test). `counted`
 // counts the pairs that reach the count.
 __device__ __forceinline__ bool pair_satisfied(const SatisfyArgs& A, const uint32_t* PU, uint32_t* rec_plane,
-                                               const uint32_t* zw_plane, const uint32_t* ubits, uint32_t plane,
-                                               uint32_t qs, uint32_t i, uint32_t& counted) {
+                                               const uint32_t* pb_plane, uint32_t plane, uint32_t qs, uint32_t i,
+                                               uint32_t& counted) {
     const DevCase& c = A.c;
     const uint32_t st1 = c.stride[1];
     const uint32_t sy = qs / c.shape[0];
@@ -972,11 +981,14 @@ __device__ __forceinline__ bool pair_satisfied(const SatisfyArgs& A, const uint3
     uint32_t* const slot = rec_plane + rec_offset(c.PQ, qs, i);
     const uint32_t rj = *slot;
     if (!(rj & kRecEscape)) {
+        // The bitmask of the block probe (the OR bitmask for a box spanning two
+        // θ and two v cells, where it is built): the pair's probe base.
+        const uint32_t* const bm = reinterpret_cast<const uint32_t*>(A.unreached_bits);
         const uint32_t ra = sx + sy * st1 + rec_off0(rj) +
-                            static_cast<uint32_t>(__mul24(rec_off1(rj), static_cast<int>(st1))) + zw_plane[i];
+                            static_cast<uint32_t>(__mul24(rec_off1(rj), static_cast<int>(st1))) + pb_plane[i];
         const uint32_t rb = ra + static_cast<uint32_t>(__mul24(static_cast<int>(rec_ext1(rj)), static_cast<int>(st1)));
         const uint32_t e = rec_ext0(rj);
-        if (bits_any(ubits[ra >> 5], ubits[(ra >> 5) + 1], ra, e) || bits_any(ubits[rb >> 5], ubits[(rb >> 5) + 1], rb, e))
+        if (bits_any(bm[ra >> 5], bm[(ra >> 5) + 1], ra, e) || bits_any(bm[rb >> 5], bm[(rb >> 5) + 1], rb, e))
             return false;
     }
     const uint32_t iw = i / c.n_a;
@@ -1138,6 +1150,19 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) void sa
 // state whose unreached count changed since the previous iteration. The
 // satisfaction workgroups read their chunks' flags first and end at once when
 // none is set, without reading their states. Padding bytes keep their zeros.
+// 64 bits of a flat bitmask from bit `pos` on. pos is a state index, and the
+// bitmask holds one word past the last state's, so both reads are in bounds.
+__device__ __forceinline__ uint64_t bits64_at(const unsigned long long* __restrict__ b, uint32_t pos) {
+    const uint32_t w = pos >> 6, sh = pos & 63u;
+    const uint64_t lo = b[w];
+    return sh ? (lo >> sh) | (static_cast<uint64_t>(b[w + 1]) << (64u - sh)) : lo;
+}
+
+// Chunk k's flag, and (where the OR bitmask is used) its word of the OR
+// bitmask: the chunk's 64 cells of valid, unreached states ORed with the same
+// cells one θ plane up, one v plane up, and both — the next θ (v) plane wrapping
+// to the first when θ (v) is the wrap axis, absent past the last otherwise.
+// Each plane's words start on a word boundary, so a chunk is exactly a word.
 __device__ __forceinline__ void chunk_flags_body(const SatisfyArgs& A, uint8_t* __restrict__ flags, uint32_t iter,
                                                  uint32_t k) {
     const DevCase& c = A.c;
@@ -1147,6 +1172,25 @@ __device__ __forceinline__ void chunk_flags_body(const SatisfyArgs& A, uint8_t* 
     const uint32_t* const CU = A.coarse_unreached[iter & 1u];
     const uint32_t* const Cprev = iter == 1 ? A.coarse_valid : A.coarse_unreached[(iter - 1u) & 1u];
     flags[plane * chunk_flag_pitch(c) + j] = chunk_changed(A, A.chunk_hull[k], plane, CU, Cprev) ? 1u : 0u;
+    if (A.probe_or_bits) {
+        const uint32_t q0 = j * 64u;
+        const uint32_t n2 = c.shape[2];
+        const uint32_t z = plane % n2, w = plane / n2;
+        const bool zl = z + 1u < n2, wl = w + 1u < c.shape[3];
+        const bool has_z = zl || c.wrap_dim == 2, has_w = wl || c.wrap_dim == 3;
+        const uint32_t pz = zl ? plane + 1u : plane - z;
+        const uint32_t pw = wl ? plane + n2 : plane - w * n2;
+        const uint32_t pzw = wl ? pz + n2 : pz - w * n2;
+        const unsigned long long* const U = A.unreached_bits;
+        const uint32_t PQ = c.PQ;
+        uint64_t m = bits64_at(U, plane * PQ + q0);
+        if (has_z) m |= bits64_at(U, pz * PQ + q0);
+        if (has_w) m |= bits64_at(U, pw * PQ + q0);
+        if (has_z && has_w) m |= bits64_at(U, pzw * PQ + q0);
+        const uint32_t left = PQ - q0;  // cells of the plane from q0 on
+        if (left < 64u) m &= (1ull << left) - 1ull;
+        const_cast<unsigned long long*>(A.probe_or_bits)[k] = m;
+    }
 }
 
 __global__ __launch_bounds__(256) void chunk_flags_kernel(SatisfyArgs A, uint8_t* __restrict__ flags,

@@ -87,7 +87,14 @@ struct SatisfyArgs {
     const uint2* hull2;     // [n_θ] union of the θ ranges over ω (lo > hi: wraps)
     const uint2* hull3;     // [n_v] union of the v ranges over a
     const unsigned long long* unreached_bits;
-    const uint32_t* probe_zw;  // [plane][I_pad]: lo2*stride2 + lo3*stride3 of the pair's box
+    // [plane][I_pad]: where the row probes of the pair's box start in the
+    // probe bitmasks, for the box's lower θ and v cells (probe_base_kernel).
+    const uint32_t* probe_base;
+    // Plane-aligned bitmask of the valid, unreached cells ORed with their θ + 1,
+    // v + 1 and θ + 1 / v + 1 neighbours, written each iteration by the chunk
+    // flag pass (chunk_flags_body); null where that pass does not run, and then
+    // no probe base points into it.
+    const unsigned long long* probe_or_bits;
     uint32_t* rec;
     const OverflowEntry* overflow;
     const uint2* box2;
@@ -158,7 +165,18 @@ __global__ void coarse_kernel(DevCase c, const uint32_t* P0, const uint32_t* P1,
 __global__ void coarse_batch_kernel(const BatchInstance* B, const uint32_t* starts, uint32_t n);
 __global__ void unreached_bits_kernel(uint32_t S, const uint8_t* valid, const uint8_t* reach, unsigned long long* bits,
                                       const IterCtl* ctl);
-__global__ void probe_zw_kernel(DevCase c, const uint2* box2, const uint2* box3, uint32_t* probe_zw);
+// Row probes (see satisfy_kernel): a row of x cells at one y, θ and v, as bits
+// [row, row + extent] of a bitmask. unreached_bits (flat state order, one word
+// past the last state's) is followed in the same allocation by probe_or_bits
+// (n_tasks + 1 words), so both are read from one base: a row starts at bit
+// x + y*n_x + plane*PQ of the first, or at bit probe_or_bit0(c) +
+// plane*probe_or_pitch(c) + x + y*n_x, where each plane starts on a word.
+__host__ __device__ inline uint32_t probe_or_pitch(const DevCase& c) { return c.chunks_per_plane * 64u; }
+__host__ __device__ inline uint32_t probe_bits_words(const DevCase& c) { return (c.S + 63u) / 64u + 1u; }
+__host__ __device__ inline uint32_t probe_or_bit0(const DevCase& c) { return probe_bits_words(c) * 64u; }
+// use_or: the OR bitmask is built (the chunk flag pass runs) for this case.
+__global__ void probe_base_kernel(DevCase c, const uint2* box2, const uint2* box3, uint32_t use_or,
+                                  uint32_t* probe_base);
 __global__ void abstraction_kernel(AbstractionArgs A);
 __global__ void export_kernel(DevCase c, const uint32_t* rec, const uint2* box2, const uint2* box3,
                               const OverflowEntry* overflow, uint64_t pair_begin, uint64_t pair_end,

@@ -168,6 +168,9 @@ void gsc_solver::setup_launch() {
     {
         const uint64_t tasks = static_cast<uint64_t>(dc.chunks_per_plane) * shape.state_shape[2] * shape.state_shape[3];
         if (tasks * 64 >= 0xFFFFFFFFull) throw ConfigError("state grid too large for one launch");
+        // Probe rows are 32-bit bit offsets over both probe bitmasks.
+        if (((static_cast<uint64_t>(dc.S) + 63) / 64 + 2 + tasks) * 64 >= 0xFFFFFFFFull)
+            throw ConfigError("state grid too large for one launch");
         dc.n_tasks = static_cast<uint32_t>(tasks);
         dc.groups_per_plane = (dc.PQ + 255) / 256;
         dc.n_groups = dc.groups_per_plane * shape.state_shape[2] * shape.state_shape[3];
@@ -233,8 +236,11 @@ void gsc_solver::setup_launch() {
     chunk_flags = alloc<uint8_t>(n_planes * chunk_flag_pitch(dc));
     // Recompute mode keeps no per-pair state at all.
     if (!opts.recompute_abstraction) rec = alloc<uint32_t>(n_planes * dc.I_pad * dc.PQ);
-    probe_zw = alloc<uint32_t>(n_planes * dc.I_pad);
-    ubits = alloc<unsigned long long>((S + 63) / 64 + 1);  // +1: the row probe reads one word past
+    probe_base = alloc<uint32_t>(n_planes * dc.I_pad);
+    // The plain probe bitmask (+1 word: the row probe reads one word past),
+    // then the plane-aligned OR bitmask (one word per chunk, +1).
+    ubits = alloc<unsigned long long>(static_cast<uint64_t>(probe_bits_words(dc)) + dc.n_tasks + 1);
+    probe_or_bits = ubits + probe_bits_words(dc);
     overflow_capacity = opts.recompute_abstraction ? 1u : 1u << 16;
     overflow = alloc<OverflowEntry>(overflow_capacity);
     overflow_count = alloc<uint32_t>(1);
@@ -335,7 +341,8 @@ uint64_t gsc_solver::device_bytes(const gsc_case_shape& sh, const gsc_case_desc&
     b += r(16 * planes * ((n0 * n1 + 63) / 64));
     b += r(planes * (((n0 * n1 + 63) / 64 + 15) / 16 * 16));
     b += 3 * r(4 * ((n0 + kCoarse - 1) / kCoarse + 1) * ((n1 + kCoarse - 1) / kCoarse + 1) * (n2 + 1) * (n3 + 1));
-    b += r(4 * planes * I_pad * n0 * n1) + r(4 * planes * I_pad) + r(8 * ((S + 63) / 64 + 1));
+    b += r(4 * planes * I_pad * n0 * n1) + r(4 * planes * I_pad);
+    b += r(8 * ((S + 63) / 64 + 1 + planes * ((n0 * n1 + 63) / 64) + 1));  // both probe bitmasks
     b += r(sizeof(OverflowEntry) * (1u << 16)) + r(4) + r(16) + r(sizeof(IterCtl)) + r(sizeof(IterRecord) * kRing);
     b += r(16 * n2 * nw * K) + r(8 * n3 * na * K) + r(8 * n2 * nw) + r(8 * n3 * na) + r(16 * n3);
     b += r(4 * (n0 + 1) * (n1 + 1)) + r(16 * planes * I * K) + r(32 * planes * I) + 2 * r(64 * 64);
@@ -416,8 +423,8 @@ void gsc_solver::upload_tables() {
     if (fail2d)
         GSC_HIP(hipMemcpyAsync(fail2d, t.fail.data(), t.fail.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
     const uint64_t n_planes = static_cast<uint64_t>(dc.shape[2]) * dc.shape[3];
-    hipLaunchKernelGGL(probe_zw_kernel, dim3(blocks_for(n_planes * dc.I_pad, 256)), dim3(256), 0, stream, dc, box2,
-                       box3, probe_zw);
+    hipLaunchKernelGGL(probe_base_kernel, dim3(blocks_for(n_planes * dc.I_pad, 256)), dim3(256), 0, stream, dc, box2,
+                       box3, probe_or_used() ? 1u : 0u, probe_base);
     GSC_HIP(hipGetLastError());
     const uint64_t n_inc = n_planes * dc.I * dc.K;
     if (n_inc) {
@@ -528,7 +535,9 @@ void gsc_solver::solve() {
     A.hull3 = hull3;
     A.prefix_valid = prefix_valid;
     A.unreached_bits = ubits;
-    A.probe_zw = probe_zw;
+    A.probe_base = probe_base;
+    A.probe_or_bits = probe_or_used() ? probe_or_bits : nullptr;
+    const bool chunk_flags_used = !opts.sweep_all_pairs && dc.hull_enabled;
     A.rec = rec;
     A.overflow = overflow;
     A.box2 = box2;
@@ -547,7 +556,6 @@ void gsc_solver::solve() {
     RA.cell_offs = cell_offs;
     RA.rad = rad;
     RA.fail2d = fail2d;
-    const bool chunk_flags_used = !opts.sweep_all_pairs && dc.hull_enabled;
     const uint32_t max_it = cfg.max_iterations;
     uint32_t done = 0;
     uint32_t finished = 0;

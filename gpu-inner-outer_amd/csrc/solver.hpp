@@ -249,7 +249,8 @@ struct gsc_solver {
     double* inc_sums = nullptr;  // [plane][input][4], see increment_sums_kernel
     int32_t* cell_offs = nullptr;  // [plane][input][8], see increment_sums_kernel
     unsigned long long* ubits = nullptr;
-    uint32_t* probe_zw = nullptr;
+    unsigned long long* probe_or_bits = nullptr;  // [n_tasks + 1], plane-aligned (see chunk_flags_body)
+    uint32_t* probe_base = nullptr;  // [plane][I_pad], see probe_base_kernel
     uint2 *box2 = nullptr, *box3 = nullptr;
     double* rad = nullptr;
     uint32_t* fail2d = nullptr;
@@ -357,6 +358,10 @@ struct gsc_solver {
     // Device bytes setup() allocates for a case of this shape (the arena
     // admission estimate; the overflow table may grow later).
     static uint64_t device_bytes(const gsc_case_shape& sh, const gsc_case_desc& cfg);
+    // The OR probe bitmask is built (by the chunk flag pass, which runs where
+    // frontier skipping does) and probed (by the stored-record satisfaction
+    // kernels). Fixed for the solver's life: its options do not change.
+    bool probe_or_used() const { return !opts.sweep_all_pairs && !opts.recompute_abstraction && dc.hull_enabled; }
 
     void build_tables(std::vector<double2>& h_cs, std::vector<double>& h_vt, std::vector<uint2>& h_box2,
                       std::vector<uint2>& h_box3, std::vector<double>& h_rad, std::vector<uint32_t>& h_fail);
