@@ -486,6 +486,45 @@ def export_section(name):
     }
 
 
+def host_solver_section(name, threads=16):
+    """The CLI's --compare mode on the same case: its multithreaded host solver
+    (the --cpu mode, host/cpu_solver.cpp), then the gfx950 path, and the exact
+    comparison of the five result arrays (main.cpp:189-211)."""
+    import subprocess
+    import tempfile
+
+    cli = os.path.join(ROOT, "gpu-inner-outer_amd", "cuda_symbolic_control_cli")
+    case = workload(name)
+    threads = min(threads, os.cpu_count() or 1)
+    with tempfile.TemporaryDirectory() as tmp:
+        cfg = os.path.join(tmp, "case.cfg")
+        with open(cfg, "w") as f:
+            f.write(case.to_cfg().replace("verbose = true", "verbose = false"))
+        env = dict(os.environ, GSC_CPU_THREADS=str(threads))
+        t0 = time.perf_counter()
+        p = subprocess.run([cli, cfg, "--compare", "--output", os.path.join(tmp, "cmp.h5")], capture_output=True,
+                           text=True, env=env, timeout=600)
+        wall = time.perf_counter() - t0
+    if p.returncode not in (0, 3):
+        raise RuntimeError(f"--compare exited {p.returncode}: {p.stderr[-300:]}")
+    # The [cpu] block's "abstraction" and "solve" lines (main.cpp:12-39).
+    cpu_block = p.stdout.split("[cpu]", 1)[1].split("[gpu]", 1)[0]
+
+    def field(label):
+        for ln in cpu_block.splitlines():
+            if ln.startswith(label):
+                return float(ln.split(":", 1)[1].split()[0])
+        return None
+
+    ab, so = field("abstraction"), field("solve")
+    return {
+        "workload": case.name, "threads": threads, "abstraction_ms": ab, "solve_ms": so,
+        "total_ms": None if ab is None or so is None else ab + so,
+        "compare_wall_s": wall, "exact_match": "exact_match   : true" in p.stdout,
+        "reference_cpu_ms": (_golden(GOLDEN_OF.get(name, "")) or {}).get("reference_cpu_ms"),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -598,6 +637,8 @@ def main():
         line["schedules_and_workloads"] = extra
         print("[bench] section export", file=sys.stderr, flush=True)
         line["abstraction_export"] = export_section(args.workload)
+        print("[bench] section host solver", file=sys.stderr, flush=True)
+        line["host_solver"] = host_solver_section(args.workload)
     if not args.no_batching:
         line["batching"] = batching_section()
     if not args.no_serving:

@@ -23,7 +23,10 @@
 //    reached ⊆ valid), counted on u32 4D prefix tables (prefix_sum.hpp:15-63,
 //    query_box_count :94-112). States are processed in parallel; each thread
 //    writes only its own states, and every count reads the iteration's
-//    snapshot, so the update order does not matter.
+//    snapshot, so the update order does not matter. Before a pair's count,
+//    three rows of its box are probed in a bitmask of the valid unreached
+//    cells (0.5 MB at 4M states, cache-resident): a set bit proves the pair
+//    unsatisfied, which settles almost every pair without the count.
 //  - Frontier skipping: a state that was tested and not satisfied keeps that
 //    answer until a cell of one of its boxes turns from unreached to reached.
 //    Each state keeps the union of its valid boxes (its hull); when the
@@ -613,6 +616,78 @@ inline std::uint32_t box_count(const Case& c, const std::uint32_t* P, const std:
     return left + box_sum(P, c.ps, l2, h2);
 }
 
+// Row probes: one bit per state (flat order), 1 for a valid, unreached state,
+// and the same ORed with each state's θ + 1, v + 1 and θ + 1 / v + 1 neighbours
+// (the next θ (v) cell wrapping to the first when θ (v) is the wrap axis, none
+// past the last otherwise). A set bit in a box's x-range on one of its rows
+// proves the box holds a valid unreached cell; the OR bitmask is read for boxes
+// that span two θ and two v cells, where its four slices lie inside the box.
+struct ProbeBits {
+    std::vector<std::uint64_t> plain, ored;  // (S + 63) / 64 + 1 words each
+};
+
+void build_probe_bits(const Case& c, const std::vector<std::uint8_t>& valid, const std::vector<std::uint8_t>& reach,
+                      ProbeBits& pb, unsigned threads) {
+    const std::uint64_t words = (c.S + 63) / 64 + 1;
+    pb.plain.assign(words, 0);
+    pb.ored.assign(words, 0);
+    parallel_for(threads, c.S / 64 + 1, 256, [&](std::uint64_t b, std::uint64_t e) {
+        for (std::uint64_t w = b; w < e; ++w) {
+            std::uint64_t m = 0;
+            for (std::uint64_t s = w * 64, k = 0; k < 64 && s < c.S; ++k, ++s)
+                m |= static_cast<std::uint64_t>(valid[s] && !reach[s]) << k;
+            pb.plain[w] = m;
+        }
+    });
+    const std::uint64_t st2 = c.PQ, st3 = static_cast<std::uint64_t>(c.PQ) * c.n[2];
+    auto bit = [&](std::uint64_t s) { return (pb.plain[s >> 6] >> (s & 63)) & 1u; };
+    parallel_for(threads, c.S / 64 + 1, 256, [&](std::uint64_t b, std::uint64_t e) {
+        for (std::uint64_t w = b; w < e; ++w) {
+            std::uint64_t m = 0;
+            for (std::uint64_t s = w * 64, k = 0; k < 64 && s < c.S; ++k, ++s) {
+                const std::uint64_t plane = s / st2;
+                const std::uint32_t z = static_cast<std::uint32_t>(plane % c.n[2]);
+                const std::uint32_t v = static_cast<std::uint32_t>(plane / c.n[2]);
+                const bool zl = z + 1 < c.n[2], vl = v + 1 < c.n[3];
+                const bool has_z = zl || c.wd == 2, has_v = vl || c.wd == 3;
+                const std::uint64_t sz = zl ? s + st2 : s - z * st2;
+                const std::uint64_t sv = vl ? s + st3 : s - v * st3;
+                const std::uint64_t szv = vl ? sz + st3 : sz - v * st3;
+                std::uint64_t u = bit(s);
+                if (has_z) u |= bit(sz);
+                if (has_v) u |= bit(sv);
+                if (has_z && has_v) u |= bit(szv);
+                m |= u << k;
+            }
+            pb.ored[w] = m;
+        }
+    });
+}
+
+// Any set bit in [pos, pos + ext] (ext < 64).
+inline bool bits_any(const std::vector<std::uint64_t>& b, std::uint64_t pos, std::uint32_t ext) {
+    const std::uint64_t w = pos >> 6, sh = pos & 63;
+    std::uint64_t v = b[w] >> sh;
+    if (sh) v |= b[w + 1] << (64 - sh);
+    const std::uint64_t mask = ext >= 63 ? ~0ull : ((1ull << (ext + 1)) - 1);
+    return (v & mask) != 0;
+}
+
+// A valid unreached cell seen on the box's middle, first or last y row at its
+// lower θ and v (the four θ / v corner slices when the box spans them).
+inline bool probe_unsatisfied(const Case& c, const ProbeBits& pb, const std::uint32_t lo[4],
+                              const std::uint32_t hi[4]) {
+    if (lo[0] > hi[0] || lo[1] > hi[1] || hi[0] - lo[0] >= 64) return false;
+    const bool span = lo[2] != hi[2] && lo[3] != hi[3];
+    const std::vector<std::uint64_t>& b = span ? pb.ored : pb.plain;
+    const std::uint64_t base = lo[0] + static_cast<std::uint64_t>(c.PQ) * (lo[2] + static_cast<std::uint64_t>(c.n[2]) * lo[3]);
+    const std::uint32_t ext = hi[0] - lo[0];
+    const std::uint32_t ym = lo[1] + ((hi[1] - lo[1]) >> 1);
+    return bits_any(b, base + static_cast<std::uint64_t>(ym) * c.n[0], ext) ||
+           bits_any(b, base + static_cast<std::uint64_t>(lo[1]) * c.n[0], ext) ||
+           bits_any(b, base + static_cast<std::uint64_t>(hi[1]) * c.n[0], ext);
+}
+
 }  // namespace
 
 unsigned default_cpu_threads() {
@@ -675,6 +750,7 @@ CpuRunReport run_case_cpu_u32(const CaseConfig& cfg, const CaseExtensions& ext, 
         }
 
     std::vector<std::uint32_t> pv, pu[2];
+    ProbeBits pb;
     build_prefix(c, valid, nullptr, pv, threads);
     std::uint64_t certified = 0;
     const auto t0 = Clock::now();
@@ -684,6 +760,7 @@ CpuRunReport run_case_cpu_u32(const CaseConfig& cfg, const CaseExtensions& ext, 
     for (std::uint32_t iter = 1; iter <= cfg.max_iterations; ++iter) {
         std::vector<std::uint32_t>& cur = pu[iter & 1];
         build_prefix(c, valid, reach.data(), cur, threads);
+        build_probe_bits(c, valid, reach, pb, threads);
         const std::uint32_t* U = cur.data();
         const std::uint32_t* Uprev = iter == 1 ? pv.data() : pu[(iter + 1) & 1].data();
         const std::uint32_t* V = pv.data();
@@ -721,6 +798,7 @@ CpuRunReport run_case_cpu_u32(const CaseConfig& cfg, const CaseExtensions& ext, 
                     const std::uint64_t xrow = (static_cast<std::uint64_t>(p) * n0 + ix) * c.I;
                     for (std::uint32_t i = 0; i < c.I; ++i) {
                         if (!pair_cells(c, a, pr, xrow, yrow, i, lo, hi)) continue;
+                        if (probe_unsatisfied(c, pb, lo, hi)) continue;
                         if (box_count(c, U, lo, hi) != 0) continue;
                         if (box_count(c, V, lo, hi) == 0) continue;
                         // reachability.hpp:89-104
