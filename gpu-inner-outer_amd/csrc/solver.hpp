@@ -374,7 +374,7 @@ struct gsc_solver {
     void build_coarse(const uint32_t* P0, const uint32_t* P1, uint32_t* C0, uint32_t* C1, const IterCtl* ctl_or_null);
     void upload_tables();         // model tables, host and device, enqueued on `stream`
     void build_abstraction();     // launch_abstraction() + finish_abstraction()
-    void launch_abstraction();    // enqueue on `stream`, no wait
+    void launch_abstraction(bool rerun = false);  // enqueue on `stream`, no wait
     void finish_abstraction();    // wait; rerun once with a larger overflow table if needed
     void seed_async();            // fixpoint seeds and control reset, enqueued on `stream`
     void solve();
