@@ -638,7 +638,10 @@ def main():
         print("[bench] section export", file=sys.stderr, flush=True)
         line["abstraction_export"] = export_section(args.workload)
         print("[bench] section host solver", file=sys.stderr, flush=True)
-        line["host_solver"] = host_solver_section(args.workload)
+        try:
+            line["host_solver"] = host_solver_section(args.workload)
+        except Exception as e:  # the CLI or its HDF5 library unavailable: the reason is kept
+            line["host_solver"] = {"error": str(e)[:300]}
     if not args.no_batching:
         line["batching"] = batching_section()
     if not args.no_serving:
