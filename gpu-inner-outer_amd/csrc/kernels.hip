@@ -556,33 +556,94 @@ __global__ __launch_bounds__(256) void increment_sums_kernel(DevCase c, const do
     }
 }
 
-// θ/v part of the probe row per (plane, input): the bit where the plane
-// lo2 + n_θ * lo3 of the pair's successor box starts in the probe bitmasks. A
-// box whose θ range holds lo2 and the next θ cell (the range is longer than one
-// cell, or wraps) and whose v range holds lo3 and the next v cell probes the OR
-// bitmask, which covers those four slices, when it is built (use_or). 0 for
-// padding inputs and rejected ranges, whose records are invalid and never
-// probed.
-__global__ __launch_bounds__(256) void probe_base_kernel(DevCase c, const uint2* __restrict__ box2,
-                                                          const uint2* __restrict__ box3, uint32_t use_or,
-                                                          uint32_t* __restrict__ probe_base) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n = c.shape[2] * c.shape[3] * c.I_pad;
-    if (t >= n) return;
-    const uint32_t plane = t / c.I_pad;
-    const uint32_t i = t - plane * c.I_pad;
-    uint32_t v = 0;
-    if (i < c.I) {
-        const uint32_t iw = i / c.n_a;
-        const uint2 b2 = box2[(plane % c.shape[2]) * c.n_w + iw];
-        const uint2 b3 = box3[(plane / c.shape[2]) * c.n_a + (i - iw * c.n_a)];
-        if (b2.x != kBoxReject && b3.x != kBoxReject) {
-            const uint32_t pz = b2.x + c.shape[2] * b3.x;
-            const bool span = b2.y != b2.x && b3.y != b3.x;
-            v = use_or && span ? probe_or_bit0(c) + pz * probe_or_pitch(c) : pz * c.PQ;
+// Per (plane, input) of the padded input range, once per abstraction, after
+// increment_sums_kernel:
+//   probe_base  θ/v part of the probe row: the bit where the plane
+//               lo2 + n_θ * lo3 of the pair's successor box starts in the probe
+//               bitmasks. A box whose θ range holds lo2 and the next θ cell (the
+//               range is longer than one cell, or wraps) and whose v range holds
+//               lo3 and the next v cell probes the OR bitmask, which covers those
+//               four slices, when it is built (use_or). 0 for padding inputs and
+//               rejected ranges, whose records are invalid and never probed.
+//   row_kind    how the abstraction kernel builds the row's records (kRow*).
+//   row_box     the θ and v ranges (inputs below I only).
+//   fast_offs   FastRow of the row: its cell offsets, and the cells whose box
+//               stays in the domain as a start and a span; a row with no such
+//               cell (an invalid row, a padding input) gets start kFastNoCell
+//               and span 0, so that no cell passes.
+// and per plane, plane_mixed: 0 when the abstraction kernel can build every
+// row of the plane as a fast row, with the constraint rectangle test on every
+// row when the case has a constraint (the rows are kRowFastChecked or
+// kRowInvalid), without it otherwise (kRowFast or kRowInvalid); else 1.
+// A row is kRowFast when every one of its pairs follows from the row's cell
+// offsets alone: the offsets decided both axes, the x/y wrap is off, the θ/v
+// ranges are ordered or wrap on their own axis, and the unclamped offsets and
+// extents fit the record fields. Clamping a cell's range to the grid moves
+// each edge offset toward 0 and shortens the extent, so the clamped box of
+// every cell fits as well. Under a constraint the row also needs the rectangle
+// test unless the reference's constraint walk is empty for it (an inverted v
+// range, or an inverted θ range that does not wrap), then kRowFastChecked.
+// One wave per plane.
+__global__ __launch_bounds__(256) void pair_rows_kernel(DevCase c, const uint2* __restrict__ box2,
+                                                         const uint2* __restrict__ box3,
+                                                         const int32_t* __restrict__ offs, uint32_t use_or,
+                                                         uint32_t* __restrict__ probe_base,
+                                                         uint32_t* __restrict__ row_kind, uint4* __restrict__ row_box,
+                                                         FastRow* __restrict__ fast_offs,
+                                                         uint32_t* __restrict__ plane_mixed) {
+    const uint32_t plane = blockIdx.x * 4u + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (plane >= c.shape[2] * c.shape[3]) return;
+    const uint32_t lane = lane_id();
+    const uint32_t iz = plane % c.shape[2], iv = plane / c.shape[2];
+    const int wd = c.wrap_dim;
+    const bool constrained = c.constraint_type != GSC_CONSTRAINT_NONE;
+    bool mixed = false;
+    for (uint32_t i = lane; i < c.I_pad; i += 64) {
+        uint32_t v = 0, kind = kRowInvalid;
+        FastRow f{{0, 0}, {0, 0}, {kFastNoCell, kFastNoCell}, {0, 0}};
+        if (i < c.I) {
+            const uint32_t iw = i / c.n_a;
+            const uint2 b2 = box2[iz * c.n_w + iw];
+            const uint2 b3 = box3[iv * c.n_a + (i - iw * c.n_a)];
+            row_box[static_cast<size_t>(plane) * c.I + i] = make_uint4(b2.x, b2.y, b3.x, b3.y);
+            if (b2.x != kBoxReject && b3.x != kBoxReject) {
+                const uint32_t pz = b2.x + c.shape[2] * b3.x;
+                const bool span = b2.y != b2.x && b3.y != b3.x;
+                v = use_or && span ? probe_or_bit0(c) + pz * probe_or_pitch(c) : pz * c.PQ;
+                const int32_t* const o = offs + (static_cast<size_t>(plane) * c.I + i) * 8;
+                const bool inv2 = b2.x > b2.y, inv3 = b3.x > b3.y;
+                bool fast = wd >= 2 && (!inv2 || wd == 2) && (!inv3 || wd == 3);
+                for (int d = 0; d < 2; ++d) {
+                    const int32_t lo = o[4 * d], hi = o[4 * d + 1];
+                    fast = fast && lo != kOffsetsUndecided && lo >= kOffMin && hi <= kOffMax && lo <= hi &&
+                           hi - lo <= static_cast<int32_t>(kExtMax);
+                }
+                const bool empty = inv3 || (inv2 && wd != 2);
+                kind = !fast ? kRowGeneral : constrained && !empty ? kRowFastChecked : kRowFast;
+                if (fast) {
+                    // Cells i with rej_lo <= i <= rej_hi, within [0, n - 1].
+                    bool any_cell = true;
+                    for (int d = 0; d < 2; ++d) {
+                        const int32_t n = static_cast<int32_t>(c.shape[d]);
+                        const int32_t a = max(o[4 * d + 2], 0), b = min(o[4 * d + 3], n - 1);
+                        f.off_lo[d] = o[4 * d];
+                        f.off_hi[d] = o[4 * d + 1];
+                        f.start[d] = a;
+                        f.span[d] = static_cast<uint32_t>(b - a);
+                        any_cell = any_cell && a <= b;
+                    }
+                    if (!any_cell) f = FastRow{{0, 0}, {0, 0}, {kFastNoCell, kFastNoCell}, {0, 0}};
+                }
+            }
         }
+        const size_t t = static_cast<size_t>(plane) * c.I_pad + i;
+        probe_base[t] = v;
+        row_kind[t] = kind;
+        fast_offs[t] = f;
+        mixed |= kind == kRowGeneral || (constrained && kind == kRowFast);
     }
-    probe_base[t] = v;
+    const bool any = __ballot(mixed) != 0;
+    if (lane == 0) plane_mixed[plane] = any ? 1u : 0u;
 }
 
 // The kernel's own argument block, addressed as ordinary (global) memory, so
@@ -742,6 +803,100 @@ __device__ __forceinline__ uint32_t escape_record(const AbstractionArgs* __restr
     return kRecEscape | e;
 }
 
+// Median of three signed integers in one instruction (a is wave-uniform).
+__device__ __forceinline__ int32_t med3_i32(int32_t a, int32_t b, int32_t c) {
+    int32_t r;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "s"(a), "v"(b), "v"(c));
+    return r;
+}
+
+// A lane's running hull of its state's valid boxes (see gsc_device.hpp).
+struct LaneHull {
+    int x0, x1, y0, y1;
+    bool any_valid, unrepresentable;
+};
+
+// A lane's cell as the fast rows use it: a range clamped to the grid and taken
+// relative to the cell is clamp(i + off, 0, n - 1) - i = med3(off, -i, n - 1 - i).
+struct FastLane {
+    int32_t ix, iy, nix, tix, niy, tiy;
+    uint32_t fail_row_shift;
+    const char* fail_bytes;
+    bool state_ok;
+};
+
+__device__ __forceinline__ FastLane fast_lane(const DevCase& c, const uint32_t* fail2d, uint32_t ix, uint32_t iy,
+                                              bool state_ok) {
+    FastLane L;
+    L.ix = static_cast<int32_t>(ix);
+    L.iy = static_cast<int32_t>(iy);
+    L.nix = -L.ix;
+    L.tix = static_cast<int32_t>(c.shape[0]) - 1 - L.ix;
+    L.niy = -L.iy;
+    L.tiy = static_cast<int32_t>(c.shape[1]) - 1 - L.iy;
+    L.fail_row_shift = c.fail_shift + 2;
+    L.fail_bytes = reinterpret_cast<const char*>(fail2d);
+    L.state_ok = state_ok;
+    return L;
+}
+
+// The record of one pair of a fast row `o` (kRowFast, or kRowFastChecked with
+// `checked`): pair_box's offsets path and the record packing, reduced to the
+// clamps, as every box of the row fits a record; the hull takes valid boxes.
+// A row no cell qualifies for (FastRow) gives kRecInvalid on every lane.
+__device__ __forceinline__ uint32_t fast_record(const ConstPtr<int32_t> o, bool checked, const FastLane& L,
+                                                LaneHull& H) {
+    // o: off_lo x, y, off_hi x, y, start x, y, span x, y.
+    const int32_t d0 = med3_i32(o[0], L.nix, L.tix), e0 = med3_i32(o[2], L.nix, L.tix);
+    const int32_t d1 = med3_i32(o[1], L.niy, L.tiy), e1 = med3_i32(o[3], L.niy, L.tiy);
+    bool ok = L.state_ok & (static_cast<uint32_t>(L.ix - o[4]) <= static_cast<uint32_t>(o[6])) &
+              (static_cast<uint32_t>(L.iy - o[5]) <= static_cast<uint32_t>(o[7]));
+    if (checked) {
+        // pair_box's constraint rectangle test; the clamped cells keep the
+        // four reads in bounds on every lane.
+        const uint32_t r0 = static_cast<uint32_t>(L.iy + d1) << L.fail_row_shift;
+        const uint32_t r1 = static_cast<uint32_t>(L.iy + e1 + 1) << L.fail_row_shift;
+        const uint32_t c0 = static_cast<uint32_t>(L.ix + d0) << 2;
+        const uint32_t c1 = static_cast<uint32_t>(L.ix + e0 + 1) << 2;
+        const char* const fb = L.fail_bytes;
+        const auto at = [fb](uint32_t off) { return *reinterpret_cast<const uint32_t*>(fb + off); };
+        const uint32_t f = at(r1 + c1) - at(r1 + c0) - at(r0 + c1) + at(r0 + c0);
+        ok = ok & (f == 0);
+    }
+    H.any_valid |= ok;
+    if (ok) {
+        H.x0 = min(H.x0, d0);
+        H.x1 = max(H.x1, e0);
+        H.y0 = min(H.y0, d1);
+        H.y1 = max(H.y1, e1);
+    }
+    return ok ? rec_pack(d0, static_cast<uint32_t>(e0 - d0), d1, static_cast<uint32_t>(e1 - d1), false)
+              : kRecInvalid;
+}
+
+// One lane's record block (inputs i0 .. i0 + 7 of cell q) as two 16-byte stores.
+constexpr uint32_t kAbsGroup = 8;
+__device__ __forceinline__ void store_block(uint32_t* rec_plane, uint32_t PQ, uint32_t q, uint32_t i0,
+                                            const uint32_t (&rr)[kAbsGroup]) {
+    uint4* const dst = reinterpret_cast<uint4*>(rec_plane + rec_offset(PQ, q, i0));
+    dst[0] = make_uint4(rr[0], rr[1], rr[2], rr[3]);
+    dst[1] = make_uint4(rr[4], rr[5], rr[6], rr[7]);
+}
+
+// A plane whose rows all build as fast rows (plane_mixed 0): no per-row kind,
+// every row of the padded range through fast_record, kChecked for all of them
+// under a constraint.
+template <bool kChecked>
+__device__ __forceinline__ void fast_plane(ConstPtr<int32_t> rows, uint32_t I, bool put, uint32_t* rec_plane,
+                                           uint32_t PQ, uint32_t q, const FastLane& L, LaneHull& H) {
+    for (uint32_t i0 = 0; i0 < I; i0 += kAbsGroup, rows += 8 * kAbsGroup) {
+        uint32_t rr[kAbsGroup];
+#pragma unroll
+        for (uint32_t j = 0; j < kAbsGroup; ++j) rr[j] = fast_record(rows + 8 * j, kChecked, L, H);
+        if (put) store_block(rec_plane, PQ, q, i0, rr);
+    }
+}
+
 // Abstraction: every pair's box, packed into its record (see gsc_device.hpp).
 // The `dead` bit of compact records is left 0 here and settled lazily by the
 // satisfaction pass (see satisfy_kernel); escaped pairs get it eagerly.
@@ -752,7 +907,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void a
     if (task >= c.n_tasks) return;
     uint32_t plane, q0;
     wave_task(c, task, &plane, &q0);
-    const uint32_t iz = plane % c.shape[2];
     const uint32_t iv = plane / c.shape[2];
     const uint32_t q = q0 + lane;
     const bool live = q < c.PQ;
@@ -767,83 +921,88 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void a
     uint32_t* const rec_plane = A.rec + static_cast<uint64_t>(plane) * c.I_pad * c.PQ;
     const bool store = A.rec != nullptr;  // false: state hulls only
     // Running hull of the state's valid boxes (see gsc_device.hpp).
-    int hx0 = kHullMaxOffset + 1, hx1 = -kHullMaxOffset - 1, hy0 = kHullMaxOffset + 1, hy1 = -kHullMaxOffset - 1;
-    bool any_valid = false, unrepresentable = false;
-    // The θ / v ranges are wave-uniform per input: scalar loads through the
-    // constant address space (a plain load could alias the record stores
-    // below, and would be a vector load waited for on every input).
-    const ConstPtr<uint32_t> box2 = to_const(reinterpret_cast<const uint32_t*>(A.box2 + iz * c.n_w));
-    const ConstPtr<uint32_t> box3 = to_const(reinterpret_cast<const uint32_t*>(A.box3 + iv * c.n_a));
+    LaneHull H{kHullMaxOffset + 1, -kHullMaxOffset - 1, kHullMaxOffset + 1, -kHullMaxOffset - 1, false, false};
+    // Per-input row data is wave-uniform: scalar loads through the constant
+    // address space (a plain load could alias the record stores below, and
+    // would be a vector load waited for on every input).
+    static_assert(kRecBlock == kAbsGroup && kInputBlock % kAbsGroup == 0, "a group is one record block");
+    const bool wave_states_ok = __ballot(state_ok) != 0;
+    const size_t row0 = static_cast<size_t>(plane) * c.I;
+    const ConstPtr<int32_t> fast_rows =
+        to_const(reinterpret_cast<const int32_t*>(A.fast_offs + static_cast<size_t>(plane) * c.I_pad));
+    const FastLane L = fast_lane(c, A.fail2d, ix, iy, state_ok);
+    const bool put = live && store;
     // Inputs a record block (8) at a time: the pairs' scalar loads and
     // arithmetic interleave, and their records leave as two 16-byte stores per
     // lane, padding inputs (I to I_pad) included as kRecInvalid.
-    constexpr uint32_t kAbsGroup = 8;
-    static_assert(kRecBlock == kAbsGroup, "a group is one record block");
-    const bool wave_states_ok = __ballot(state_ok) != 0;
-    // Per-input table rows, advanced input by input (i = iw * n_a + ia)
-    // without a division or a product.
-    ConstPtr<uint32_t> b2_row = box2, b3_row = box3;
-    ConstPtr<int32_t> offs_row = to_const(A.cell_offs + static_cast<size_t>(plane) * c.I * 8);
-    uint32_t ia_next = 0;
-    for (uint32_t i0 = 0; i0 < c.I; i0 += kAbsGroup) {
+    if (!wave_states_ok) {
+        // The wave's states all fail the constraint (or lie past the plane):
+        // no valid pair.
         uint32_t rr[kAbsGroup];
 #pragma unroll
-        for (uint32_t j = 0; j < kAbsGroup; ++j) {
-            const uint32_t i = i0 + j;
-            rr[j] = kRecInvalid;
-            if (i >= c.I) continue;
-            const ConstPtr<uint32_t> b2p = b2_row, b3p = b3_row;
-            const ConstPtr<int32_t> offs_p = offs_row;
-            offs_row += 8;
-            b3_row += 2;
-            if (++ia_next == c.n_a) {
-                ia_next = 0;
-                b3_row = box3;
-                b2_row += 2;
-            }
-            // A wave whose states all fail the constraint (or lie past the
-            // plane) has no valid pair: its records stay invalid.
-            if (!wave_states_ok) continue;
-            const uint2 b2 = make_uint2(b2p[0], b2p[1]);
-            const uint2 b3 = make_uint2(b3p[0], b3p[1]);
-            uint32_t lo[4], hi[4];
-            const double2* const inc = A.inc + (static_cast<size_t>(plane) * c.I + i) * c.K;
-            const double* const sums = A.inc_sums + (static_cast<size_t>(plane) * c.I + i) * 4;
-            // The (plane, input)'s cell offsets in one scalar load, ahead of the branches that use them.
-            int32_t offs[8];
+        for (uint32_t j = 0; j < kAbsGroup; ++j) rr[j] = kRecInvalid;
+        for (uint32_t i0 = 0; i0 < c.I; i0 += kAbsGroup)
+            if (put) store_block(rec_plane, c.PQ, q, i0, rr);
+    } else if (A.plane_mixed[plane] == 0) {
+        if (c.constraint_type != GSC_CONSTRAINT_NONE)
+            fast_plane<true>(fast_rows, c.I, put, rec_plane, c.PQ, q, L, H);
+        else
+            fast_plane<false>(fast_rows, c.I, put, rec_plane, c.PQ, q, L, H);
+    } else {
+        const ConstPtr<uint32_t> kinds = to_const(A.row_kind + static_cast<size_t>(plane) * c.I_pad);
+        const ConstPtr<int32_t> offs_rows = to_const(A.cell_offs + row0 * 8);
+        const ConstPtr<uint32_t> box_rows = to_const(reinterpret_cast<const uint32_t*>(A.row_box + row0));
+        for (uint32_t i0 = 0; i0 < c.I; i0 += kAbsGroup) {
+            uint32_t rr[kAbsGroup];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) offs[k] = offs_p[k];
-            const bool ok = pair_box(c, &kernarg_block<AbstractionArgs>()->c, inc, sums, offs, b2, b3, ix, iy, x0, y0,
-                                     rad0, rad1, A.fail2d, lo, hi) &
-                            state_ok;
-            // Per-lane outcome as selects: inverted ranges on a non-wrap axis or
-            // wrapped x/y ranges take the escape path.
-            const bool regular = (lo[0] <= hi[0]) & (lo[1] <= hi[1]) & ((b2.x <= b2.y) | (wd == 2)) &
-                                 ((b3.x <= b3.y) | (wd == 3));
-            const int off0 = static_cast<int>(lo[0]) - static_cast<int>(ix);
-            const int off1 = static_cast<int>(lo[1]) - static_cast<int>(iy);
-            const uint32_t ext0 = hi[0] - lo[0], ext1 = hi[1] - lo[1];
-            const bool fits = regular & (off0 >= kOffMin) & (off0 <= kOffMax) & (off1 >= kOffMin) &
-                              (off1 <= kOffMax) & (ext0 <= kExtMax) & (ext1 <= kExtMax);
-            const bool packed = ok & fits;
-            any_valid |= ok;
-            unrepresentable |= ok & !fits;
-            hx0 = packed ? min(hx0, off0) : hx0;
-            hx1 = packed ? max(hx1, off0 + static_cast<int>(ext0)) : hx1;
-            hy0 = packed ? min(hy0, off1) : hy0;
-            hy1 = packed ? max(hy1, off1 + static_cast<int>(ext1)) : hy1;
-            uint32_t r = packed ? rec_pack(off0, ext0, off1, ext1, false) : kRecInvalid;
-            if (store && ok && !fits)
-                r = escape_record(kernarg_block<AbstractionArgs>(), lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2],
-                                  hi[3]);
-            rr[j] = r;
-        }
-        if (live && store) {
-            uint4* const dst = reinterpret_cast<uint4*>(rec_plane + rec_offset(c.PQ, q, i0));
-            dst[0] = make_uint4(rr[0], rr[1], rr[2], rr[3]);
-            dst[1] = make_uint4(rr[4], rr[5], rr[6], rr[7]);
+            for (uint32_t j = 0; j < kAbsGroup; ++j) {
+                const uint32_t i = i0 + j;
+                const uint32_t kind = kinds[i];
+                rr[j] = kRecInvalid;
+                if (kind == kRowInvalid) continue;
+                if (kind != kRowGeneral) {
+                    rr[j] = fast_record(fast_rows + 8 * i, kind == kRowFastChecked, L, H);
+                    continue;
+                }
+                const uint2 b2 = make_uint2(box_rows[4 * i], box_rows[4 * i + 1]);
+                const uint2 b3 = make_uint2(box_rows[4 * i + 2], box_rows[4 * i + 3]);
+                uint32_t lo[4], hi[4];
+                const double2* const inc = A.inc + (row0 + i) * c.K;
+                const double* const sums = A.inc_sums + (row0 + i) * 4;
+                // The (plane, input)'s cell offsets in one scalar load, ahead of the branches that use them.
+                int32_t offs[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) offs[k] = offs_rows[8 * i + k];
+                const bool ok = pair_box(c, &kernarg_block<AbstractionArgs>()->c, inc, sums, offs, b2, b3, ix, iy, x0,
+                                         y0, rad0, rad1, A.fail2d, lo, hi) &
+                                state_ok;
+                // Per-lane outcome as selects: inverted ranges on a non-wrap axis or
+                // wrapped x/y ranges take the escape path.
+                const bool regular = (lo[0] <= hi[0]) & (lo[1] <= hi[1]) & ((b2.x <= b2.y) | (wd == 2)) &
+                                     ((b3.x <= b3.y) | (wd == 3));
+                const int off0 = static_cast<int>(lo[0]) - static_cast<int>(ix);
+                const int off1 = static_cast<int>(lo[1]) - static_cast<int>(iy);
+                const uint32_t ext0 = hi[0] - lo[0], ext1 = hi[1] - lo[1];
+                const bool fits = regular & (off0 >= kOffMin) & (off0 <= kOffMax) & (off1 >= kOffMin) &
+                                  (off1 <= kOffMax) & (ext0 <= kExtMax) & (ext1 <= kExtMax);
+                const bool packed = ok & fits;
+                H.any_valid |= ok;
+                H.unrepresentable |= ok & !fits;
+                H.x0 = packed ? min(H.x0, off0) : H.x0;
+                H.x1 = packed ? max(H.x1, off0 + static_cast<int>(ext0)) : H.x1;
+                H.y0 = packed ? min(H.y0, off1) : H.y0;
+                H.y1 = packed ? max(H.y1, off1 + static_cast<int>(ext1)) : H.y1;
+                uint32_t r = packed ? rec_pack(off0, ext0, off1, ext1, false) : kRecInvalid;
+                if (store && ok && !fits)
+                    r = escape_record(kernarg_block<AbstractionArgs>(), lo[0], lo[1], lo[2], lo[3], hi[0], hi[1],
+                                      hi[2], hi[3]);
+                rr[j] = r;
+            }
+            if (live && store) store_block(rec_plane, c.PQ, q, i0, rr);
         }
     }
+    const int hx0 = H.x0, hx1 = H.x1, hy0 = H.y0, hy1 = H.y1;
+    const bool any_valid = H.any_valid, unrepresentable = H.unrepresentable;
     if (live && A.hull) {
         uint32_t h;
         if (!any_valid)

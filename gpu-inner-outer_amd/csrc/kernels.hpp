@@ -52,13 +52,31 @@ struct IterRecord {
     unsigned long long test_slots;
 };
 
+// How the abstraction kernel builds the records of one (plane, input) row.
+constexpr uint32_t kRowGeneral = 0;      // per pair: cell offsets, filter or exact sum; escapes possible
+constexpr uint32_t kRowInvalid = 1;      // every pair invalid (θ or v range rejected; padding input)
+constexpr uint32_t kRowFast = 2;         // every box from the row's cell offsets, every box fits a record
+constexpr uint32_t kRowFastChecked = 3;  // kRowFast plus the constraint rectangle test
+
+// A fast row's cell offsets, x then y, and the cells whose box stays in the
+// domain: cell i qualifies on axis d when i - start[d] <= span[d] as unsigned
+// 32-bit values. One 32-byte scalar load.
+struct FastRow {
+    int32_t off_lo[2], off_hi[2];
+    int32_t start[2];
+    uint32_t span[2];
+};
+constexpr int32_t kFastNoCell = 0x40000000;  // start of a row no cell qualifies for (span 0)
+
 struct AbstractionArgs {
     DevCase c;
     const double2* inc;          // [plane][input][K] nominal-trajectory increments
     const double* inc_sums;      // [plane][input][4]: (S -/+ rad) / eta for x, then y (increment_sums_kernel)
     const int32_t* cell_offs;    // [plane][input][8]: CellOffsets for x, then y, or kOffsetsUndecided
-    const uint2* box2;           // [n_θ][n_ω] θ cell range (or kBoxReject)
-    const uint2* box3;           // [n_v][n_a] v cell range (or kBoxReject)
+    const uint32_t* row_kind;    // [plane][I_pad]: kRow* of each (plane, input) (pair_rows_kernel)
+    const uint4* row_box;        // [plane][input]: θ range, then v range (box2 / box3 entries)
+    const FastRow* fast_offs;    // [plane][I_pad] (pair_rows_kernel)
+    const uint32_t* plane_mixed;  // [plane]: 0 when every row builds as a fast row (pair_rows_kernel)
     const double* rad;           // [n_v][2] x / y box radius
     const uint32_t* fail2d;      // 2D prefix of constraint failures ((n1+1) rows of 1 << fail_shift), or null
     const uint32_t* prefix_valid;
@@ -88,7 +106,7 @@ struct SatisfyArgs {
     const uint2* hull3;     // [n_v] union of the v ranges over a
     const unsigned long long* unreached_bits;
     // [plane][I_pad]: where the row probes of the pair's box start in the
-    // probe bitmasks, for the box's lower θ and v cells (probe_base_kernel).
+    // probe bitmasks, for the box's lower θ and v cells (pair_rows_kernel).
     const uint32_t* probe_base;
     // Plane-aligned bitmask of the valid, unreached cells ORed with their θ + 1,
     // v + 1 and θ + 1 / v + 1 neighbours, written each iteration by the chunk
@@ -174,9 +192,11 @@ __global__ void unreached_bits_kernel(uint32_t S, const uint8_t* valid, const ui
 __host__ __device__ inline uint32_t probe_or_pitch(const DevCase& c) { return c.chunks_per_plane * 64u; }
 __host__ __device__ inline uint32_t probe_bits_words(const DevCase& c) { return (c.S + 63u) / 64u + 1u; }
 __host__ __device__ inline uint32_t probe_or_bit0(const DevCase& c) { return probe_bits_words(c) * 64u; }
-// use_or: the OR bitmask is built (the chunk flag pass runs) for this case.
-__global__ void probe_base_kernel(DevCase c, const uint2* box2, const uint2* box3, uint32_t use_or,
-                                  uint32_t* probe_base);
+// Per (plane, input): the probe base, the row kind and the θ / v ranges
+// (use_or: the OR bitmask is built, the chunk flag pass runs, for this case).
+__global__ void pair_rows_kernel(DevCase c, const uint2* box2, const uint2* box3, const int32_t* offs,
+                                 uint32_t use_or, uint32_t* probe_base, uint32_t* row_kind, uint4* row_box,
+                                 FastRow* fast_offs, uint32_t* plane_mixed);
 __global__ void abstraction_kernel(AbstractionArgs A);
 __global__ void export_kernel(DevCase c, const uint32_t* rec, const uint2* box2, const uint2* box3,
                               const OverflowEntry* overflow, uint64_t pair_begin, uint64_t pair_end,

@@ -237,6 +237,7 @@ void gsc_solver::setup_launch() {
     // Recompute mode keeps no per-pair state at all.
     if (!opts.recompute_abstraction) rec = alloc<uint32_t>(n_planes * dc.I_pad * dc.PQ);
     probe_base = alloc<uint32_t>(n_planes * dc.I_pad);
+    row_kind = alloc<uint32_t>(n_planes * dc.I_pad);
     // The plain probe bitmask (+1 word: the row probe reads one word past),
     // then the plane-aligned OR bitmask (one word per chunk, +1).
     ubits = alloc<unsigned long long>(static_cast<uint64_t>(probe_bits_words(dc)) + dc.n_tasks + 1);
@@ -276,6 +277,9 @@ void gsc_solver::setup_launch() {
     inc = alloc<double2>(n_inc);
     inc_sums = alloc<double>(4ull * dc.shape[2] * dc.shape[3] * dc.I);
     cell_offs = alloc<int32_t>(8ull * dc.shape[2] * dc.shape[3] * dc.I);
+    row_box = alloc<uint4>(static_cast<uint64_t>(dc.shape[2]) * dc.shape[3] * dc.I);
+    fast_offs = alloc<FastRow>(static_cast<uint64_t>(dc.shape[2]) * dc.shape[3] * dc.I_pad);
+    plane_mixed = alloc<uint32_t>(static_cast<uint64_t>(dc.shape[2]) * dc.shape[3]);
     d_obstacles = alloc<gsc_rect4>(obstacles.size());
     d_targets = alloc<gsc_rect4>(targets.size());
     uint8_t* d_allow = allow.empty() ? nullptr : alloc<uint8_t>(S);
@@ -341,12 +345,13 @@ uint64_t gsc_solver::device_bytes(const gsc_case_shape& sh, const gsc_case_desc&
     b += r(16 * planes * ((n0 * n1 + 63) / 64));
     b += r(planes * (((n0 * n1 + 63) / 64 + 15) / 16 * 16));
     b += 3 * r(4 * ((n0 + kCoarse - 1) / kCoarse + 1) * ((n1 + kCoarse - 1) / kCoarse + 1) * (n2 + 1) * (n3 + 1));
-    b += r(4 * planes * I_pad * n0 * n1) + r(4 * planes * I_pad);
+    b += r(4 * planes * I_pad * n0 * n1) + 2 * r(4 * planes * I_pad);  // records, probe bases, row kinds
     b += r(8 * ((S + 63) / 64 + 1 + planes * ((n0 * n1 + 63) / 64) + 1));  // both probe bitmasks
     b += r(sizeof(OverflowEntry) * (1u << 16)) + r(4) + r(16) + r(sizeof(IterCtl)) + r(sizeof(IterRecord) * kRing);
     b += r(16 * n2 * nw * K) + r(8 * n3 * na * K) + r(8 * n2 * nw) + r(8 * n3 * na) + r(16 * n3);
     b += r(4 * (n0 + 1) * (n1 + 1)) + r(16 * planes * I * K) + r(32 * planes * I) + 2 * r(64 * 64);
-    b += r(32 * planes * I);  // cell offsets
+    b += r(32 * planes * I) + r(16 * planes * I);  // cell offsets, row boxes
+    b += r(sizeof(FastRow) * planes * I_pad) + r(4 * planes);  // fast rows, plane flags
     if (cfg.state_allow) b += r(S);
     return b;
 }
@@ -423,9 +428,6 @@ void gsc_solver::upload_tables() {
     if (fail2d)
         GSC_HIP(hipMemcpyAsync(fail2d, t.fail.data(), t.fail.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
     const uint64_t n_planes = static_cast<uint64_t>(dc.shape[2]) * dc.shape[3];
-    hipLaunchKernelGGL(probe_base_kernel, dim3(blocks_for(n_planes * dc.I_pad, 256)), dim3(256), 0, stream, dc, box2,
-                       box3, probe_or_used() ? 1u : 0u, probe_base);
-    GSC_HIP(hipGetLastError());
     const uint64_t n_inc = n_planes * dc.I * dc.K;
     if (n_inc) {
         hipLaunchKernelGGL(increments_kernel, dim3(blocks_for(n_inc, 256)), dim3(256), 0, stream, dc, cs, vt, inc);
@@ -435,6 +437,10 @@ void gsc_solver::upload_tables() {
     const uint64_t n_rows = n_planes * dc.I;
     hipLaunchKernelGGL(increment_sums_kernel, dim3(blocks_for(n_rows, 256)), dim3(256), 0, stream, dc, inc, rad,
                        inc_sums, cell_offs);
+    GSC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(pair_rows_kernel, dim3(blocks_for(n_planes, 4)), dim3(256), 0, stream, dc, box2, box3,
+                       cell_offs, probe_or_used() ? 1u : 0u, probe_base, row_kind, row_box, fast_offs,
+                       plane_mixed);
     GSC_HIP(hipGetLastError());
 }
 
@@ -454,8 +460,10 @@ void gsc_solver::launch_abstraction(bool rerun) {
     A.inc = inc;
     A.inc_sums = inc_sums;
     A.cell_offs = cell_offs;
-    A.box2 = box2;
-    A.box3 = box3;
+    A.row_kind = row_kind;
+    A.row_box = row_box;
+    A.fast_offs = fast_offs;
+    A.plane_mixed = plane_mixed;
     A.rad = rad;
     A.fail2d = fail2d;
     A.prefix_valid = prefix_valid;

@@ -250,7 +250,11 @@ struct gsc_solver {
     int32_t* cell_offs = nullptr;  // [plane][input][8], see increment_sums_kernel
     unsigned long long* ubits = nullptr;
     unsigned long long* probe_or_bits = nullptr;  // [n_tasks + 1], plane-aligned (see chunk_flags_body)
-    uint32_t* probe_base = nullptr;  // [plane][I_pad], see probe_base_kernel
+    uint32_t* probe_base = nullptr;  // [plane][I_pad], see pair_rows_kernel
+    uint32_t* row_kind = nullptr;    // [plane][I_pad], see pair_rows_kernel
+    uint4* row_box = nullptr;        // [plane][input], see pair_rows_kernel
+    FastRow* fast_offs = nullptr;      // [plane][I_pad], see pair_rows_kernel
+    uint32_t* plane_mixed = nullptr;   // [plane], see pair_rows_kernel
     uint2 *box2 = nullptr, *box3 = nullptr;
     double* rad = nullptr;
     uint32_t* fail2d = nullptr;
