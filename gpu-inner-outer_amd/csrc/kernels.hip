@@ -900,6 +900,10 @@ __device__ __forceinline__ void fast_plane(ConstPtr<int32_t> rows, uint32_t I, b
 // Abstraction: every pair's box, packed into its record (see gsc_device.hpp).
 // The `dead` bit of compact records is left 0 here and settled lazily by the
 // satisfaction pass (see satisfy_kernel); escaped pairs get it eagerly.
+// Two instantiations, launched one after the other over the same waves: kFast
+// builds the planes whose rows are all fast rows (plane_mixed 0), the other
+// every other plane, so each is compiled for its own loop alone.
+template <bool kFast>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void abstraction_kernel(AbstractionArgs A) {
     const DevCase& c = A.c;
     const uint32_t lane = lane_id();
@@ -907,6 +911,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void a
     if (task >= c.n_tasks) return;
     uint32_t plane, q0;
     wave_task(c, task, &plane, &q0);
+    if ((A.plane_mixed[plane] == 0) != kFast) return;
     const uint32_t iv = plane / c.shape[2];
     const uint32_t q = q0 + lane;
     const bool live = q < c.PQ;
@@ -943,7 +948,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void a
         for (uint32_t j = 0; j < kAbsGroup; ++j) rr[j] = kRecInvalid;
         for (uint32_t i0 = 0; i0 < c.I; i0 += kAbsGroup)
             if (put) store_block(rec_plane, c.PQ, q, i0, rr);
-    } else if (A.plane_mixed[plane] == 0) {
+    } else if (kFast) {
         if (c.constraint_type != GSC_CONSTRAINT_NONE)
             fast_plane<true>(fast_rows, c.I, put, rec_plane, c.PQ, q, L, H);
         else
@@ -1703,6 +1708,13 @@ hipError_t launch_satisfy(const SatisfyArgs& A, uint32_t iter, uint32_t wg, uint
                         : wg == 512 ? pick_single<512>(ib, sw)
                                     : pick_single<256>(ib, sw);
     hipLaunchKernelGGL(f, dim3(satisfy_blocks(A.c, wg)), dim3(wg), 0, stream, A, iter);
+    return hipGetLastError();
+}
+
+hipError_t launch_abstraction_kernels(const AbstractionArgs& A, hipStream_t stream) {
+    const dim3 grid((A.c.n_tasks + 3u) / 4u);
+    hipLaunchKernelGGL(abstraction_kernel<true>, grid, dim3(256), 0, stream, A);
+    hipLaunchKernelGGL(abstraction_kernel<false>, grid, dim3(256), 0, stream, A);
     return hipGetLastError();
 }
 

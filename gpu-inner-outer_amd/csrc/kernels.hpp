@@ -197,7 +197,8 @@ __host__ __device__ inline uint32_t probe_or_bit0(const DevCase& c) { return pro
 __global__ void pair_rows_kernel(DevCase c, const uint2* box2, const uint2* box3, const int32_t* offs,
                                  uint32_t use_or, uint32_t* probe_base, uint32_t* row_kind, uint4* row_box,
                                  FastRow* fast_offs, uint32_t* plane_mixed);
-__global__ void abstraction_kernel(AbstractionArgs A);
+// The abstraction kernels (fast planes, then the others) over every wave task.
+hipError_t launch_abstraction_kernels(const AbstractionArgs& A, hipStream_t stream);
 __global__ void export_kernel(DevCase c, const uint32_t* rec, const uint2* box2, const uint2* box3,
                               const OverflowEntry* overflow, uint64_t pair_begin, uint64_t pair_end,
                               uint32_t* out_min, uint32_t* out_max, uint8_t* out_valid);

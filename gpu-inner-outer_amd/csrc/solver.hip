@@ -473,8 +473,7 @@ void gsc_solver::launch_abstraction(bool rerun) {
     A.overflow_capacity = overflow_capacity;
     A.hull = hull;
     A.chunk_hull = chunk_hull;
-    hipLaunchKernelGGL(abstraction_kernel, dim3(blocks_for(dc.n_tasks, 4)), dim3(256), 0, stream, A);
-    GSC_HIP(hipGetLastError());
+    GSC_HIP(launch_abstraction_kernels(A, stream));
     if (!events.empty()) GSC_HIP(hipEventRecord(events[1], stream));
     GSC_HIP(hipMemcpyAsync(host_words + kWordOverflow, overflow_count, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
 }
