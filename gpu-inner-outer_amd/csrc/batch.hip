@@ -215,7 +215,7 @@ void gsc_batch::plan() {
         b.sat.chunk_flags = s.chunk_flags;
         b.plane_pitch = s.plane_pitch;
         fused = fused && s.plane_pitch != 0;
-        lds = std::max<uint64_t>(lds, static_cast<uint64_t>(s.plane_pitch) * c.shape[1] * sizeof(uint32_t));
+        lds = std::max<uint64_t>(lds, plane_lds_bytes(s.plane_pitch, c.shape[1], c.PQ));
         b.sat.hull2 = s.hull2;
         b.sat.hull3 = s.hull3;
         b.sat.prefix_valid = s.prefix_valid;

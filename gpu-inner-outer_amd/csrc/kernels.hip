@@ -313,16 +313,23 @@ __global__ __launch_bounds__(256) void prefix_axis_kernel(DevCase c, int axis, u
 // Fused prefix passes (planes that fit in LDS): three launches per table build
 // instead of rows, three axes, coarse sample and bitmask.
 //
-// prefix_plane_kernel: one 1024-thread workgroup per (θ, v) plane. It scans
-// valid ∧ ¬reached along x (a wave per row, straight from the masks) and then
-// along y (a wave per column) in LDS, and writes the plane's 2D prefix into the
-// padded table; the θ and v passes follow (prefix_axis_kernel with axis 2,
-// prefix_last_axis_kernel). It also writes the plane's words of the unreached
-// bitmask: word k belongs to the plane holding state 64k and is read straight
-// from the masks, including any states of the next plane it covers.
+// prefix_plane_kernel: one 1024-thread workgroup per (θ, v) plane.
+//  1. Stage: the workgroup reads the masks of states [lo, hi) as 4-byte
+//     words, every load issued before the first is used, and keeps one bit
+//     per state (valid ∧ ¬reached) in LDS; lo is the plane's first state
+//     rounded down to 64, hi covers the plane and every unreached-bitmask
+//     word that starts in it.
+//  2. The plane's words of the unreached bitmask (word k belongs to the plane
+//     holding state 64k, including any states of the next plane it covers)
+//     are copied out of the staged bits.
+//  3. x: a wave per row, 64 cells at a time; a cell's running count is the
+//     row's carry plus the set bits below it in the chunk's ballot (mbcnt).
+//  4. y: a wave per column in LDS, then the plane's 2D prefix goes into the
+//     padded table; the θ and v passes follow (prefix_axis_kernel with axis 2,
+//     prefix_last_axis_kernel).
 // LDS layout: element (x, y) at y * pitch + (x ^ (y & 63)), pitch a multiple
 // of 64, so that rows (lanes along x) and columns (lanes along y) both hit 64
-// different banks.
+// different banks; the staged bits follow (plane_stage_words).
 __device__ __forceinline__ void prefix_plane_body(const DevCase& c, const uint8_t* __restrict__ a,
                                                   const uint8_t* __restrict__ b, uint32_t* __restrict__ P,
                                                   unsigned long long* __restrict__ bits, uint32_t pitch, uint32_t plane) {
@@ -330,38 +337,81 @@ __device__ __forceinline__ void prefix_plane_body(const DevCase& c, const uint8_
     constexpr uint32_t kW = kPlaneThreads / 64;
     const uint32_t lane = lane_id();
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t n0 = c.shape[0], n1 = c.shape[1], PQ = c.PQ;
+    const uint32_t n0 = c.shape[0], n1 = c.shape[1], PQ = c.PQ, S = c.S;
     const uint32_t base = plane * PQ;
-    if (bits) {
-        const uint32_t k_begin = (base + 63u) / 64u;
-        const uint32_t k_end = min((base + PQ + 63u) / 64u, (c.S + 63u) / 64u);
-        for (uint32_t k = k_begin + wave; k < k_end; k += kW) {
-            const uint32_t s = k * 64u + lane;
-            uint32_t av = 0, bv = 0;
-            if (s < c.S) {
-                av = a[s];  // both loads issued together (no short-circuit)
-                bv = b[s];
+    uint32_t* const cb = sm + n1 * pitch;
+    const uint32_t k_begin = (base + 63u) / 64u;
+    const uint32_t k_end = min((base + PQ + 63u) / 64u, (S + 63u) / 64u);
+    const uint32_t lo = base & ~63u;
+    const uint32_t hi = max(base + PQ, k_end * 64u);  // states past S stage as 0
+    const uint32_t nd = (hi - lo + 3u) / 4u;         // staged 4-state words
+    // 1. Stage. A plane holds at most kPlaneLdsMax / 4 states, so nd fits in
+    // kStageRounds rounds of the workgroup.
+    constexpr uint32_t kStageRounds = (kPlaneLdsMax / 4u + 127u + 3u) / 4u / kPlaneThreads + 1u;
+    uint32_t av[kStageRounds], bv[kStageRounds];
+#pragma unroll
+    for (uint32_t r = 0; r < kStageRounds; ++r) {
+        const uint32_t s = lo + 4u * (r * kPlaneThreads + threadIdx.x);
+        // Whole words in the masks load as one; others read word 0 and are
+        // replaced below (S >= 4: word 0 is in the masks).
+        const uint32_t at = (s + 4u <= S) ? s : 0u;
+        av[r] = S >= 4u ? *reinterpret_cast<const uint32_t*>(a + at) : 0u;
+        bv[r] = S >= 4u && b ? *reinterpret_cast<const uint32_t*>(b + at) : 0u;
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < kStageRounds; ++r) {
+        const uint32_t i = r * kPlaneThreads + threadIdx.x;
+        const uint32_t s = lo + 4u * i;
+        uint32_t va = av[r], vb = bv[r];
+        if (s + 4u > S) {
+            // The masks end inside (or before) this word: its states below S
+            // byte by byte, the rest 0.
+            va = 0;
+            vb = 0;
+            for (uint32_t j = 0; s + j < S && j < 4u; ++j) {
+                va |= static_cast<uint32_t>(a[s + j]) << (8u * j);
+                if (b) vb |= static_cast<uint32_t>(b[s + j]) << (8u * j);
             }
-            const uint64_t m = __ballot((av != 0) & (bv == 0));
-            if (lane == 0) bits[k] = m;
+        }
+        // Bit 7 of each byte: the byte is non-zero (its low seven bits carry
+        // into bit 7, or bit 7 is set). The four cell bits a ≠ 0 ∧ b = 0
+        // sit at bits 7, 15, 23, 31; the product moves them to bits 21..24
+        // (all partial products land on distinct bits).
+        const uint32_t nz_a = (((va & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | va) & 0x80808080u;
+        const uint32_t nz_b = (((vb & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | vb) & 0x80808080u;
+        const uint32_t m = (nz_a & ~nz_b) >> 7;
+        uint32_t nib = ((m * 0x204081u) >> 21) & 0xFu;
+        if (i >= nd) nib = 0;
+        // Eight consecutive words of states (eight lanes) make one LDS word.
+        uint32_t w = nib << (4u * (lane & 7u));
+        w |= static_cast<uint32_t>(__shfl_xor(static_cast<int>(w), 1, 64));
+        w |= static_cast<uint32_t>(__shfl_xor(static_cast<int>(w), 2, 64));
+        w |= static_cast<uint32_t>(__shfl_xor(static_cast<int>(w), 4, 64));
+        if ((lane & 7u) == 0 && i < nd) cb[i / 8u] = w;
+    }
+    __syncthreads();
+    // 2. Bitmask words, straight from the staged bits (64k - lo is a multiple of 64).
+    if (bits) {
+        for (uint32_t k = k_begin + threadIdx.x; k < k_end; k += kPlaneThreads) {
+            const uint32_t wi = (k * 64u - lo) / 32u;
+            bits[k] = static_cast<unsigned long long>(cb[wi]) | (static_cast<unsigned long long>(cb[wi + 1]) << 32);
         }
     }
-    // x: a wave per row, from the masks.
+    // 3. x: a wave per row, 64 cells at a time: each lane reads its cell's
+    // staged bit, the ballot of the chunk gives each lane the count of set
+    // bits below it (mbcnt), and the row's carry adds the chunks before.
     for (uint32_t y = wave; y < n1; y += kW) {
         const uint32_t sw = y & 63u;
+        const uint32_t row_bit = base + y * n0 - lo;
         uint32_t carry = 0;
         for (uint32_t x0 = 0; x0 < n0; x0 += 64u) {
             const uint32_t x = x0 + lane;
-            const uint32_t q = base + y * n0 + x;
-            uint32_t av = 0, bv = 0;
-            if (x < n0) {
-                av = a[q];  // both loads issued together (no short-circuit)
-                bv = b ? b[q] : 0u;
-            }
-            uint32_t v = ((av != 0) & (bv == 0)) ? 1u : 0u;
-            v = wave_inclusive_scan(v) + carry;
+            const uint32_t o = row_bit + x;
+            const bool mine = (x < n0) & (((cb[o >> 5] >> (o & 31u)) & 1u) != 0u);
+            const uint64_t m = __ballot(mine);
+            const uint32_t v = carry + lanes_below(m) + (mine ? 1u : 0u);
             if (x < n0) sm[y * pitch + (x ^ sw)] = v;
-            carry = last_lane(v);
+            carry += static_cast<uint32_t>(__popcll(m));
         }
     }
     __syncthreads();

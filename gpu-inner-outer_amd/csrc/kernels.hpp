@@ -172,6 +172,12 @@ __global__ void increments_kernel(DevCase c, const double2* cs, const double* vt
 // Fused prefix build for planes that fit in LDS (see kernels.hip).
 constexpr uint32_t kPlaneThreads = 1024;
 constexpr uint32_t kPlaneLdsMax = 65536;  // bytes of LDS for one plane's 2D prefix
+// The plane kernel's LDS: the 2D prefix (pitch words per row), then one bit per
+// staged state, for the plane's PQ states plus up to 63 on either side.
+__host__ __device__ inline uint32_t plane_stage_words(uint32_t PQ) { return (PQ + 127u) / 32u + 3u; }
+__host__ __device__ inline uint32_t plane_lds_bytes(uint32_t pitch, uint32_t n1, uint32_t PQ) {
+    return (pitch * n1 + plane_stage_words(PQ)) * 4u;
+}
 __global__ void prefix_plane_kernel(DevCase c, const uint8_t* a, const uint8_t* b, uint32_t* P0, uint32_t* P1,
                                     unsigned long long* bits, uint32_t pitch, const IterCtl* ctl);
 __global__ void prefix_last_axis_kernel(DevCase c, uint32_t* P0, uint32_t* P1, uint32_t* C0, uint32_t* C1,

@@ -578,7 +578,7 @@ void gsc_solver::solve() {
             if (plane_pitch) {
                 // Plane kernel (x, y and the bitmask), θ pass, v pass with the coarse sample.
                 hipLaunchKernelGGL(prefix_plane_kernel, dim3(dc.shape[2] * dc.shape[3]), dim3(kPlaneThreads),
-                                   plane_pitch * dc.shape[1] * sizeof(uint32_t), stream, dc, valid, reach,
+                                   plane_lds_bytes(plane_pitch, dc.shape[1], dc.PQ), stream, dc, valid, reach,
                                    prefix_unreached[0], prefix_unreached[1], ubits, plane_pitch, ctl);
                 GSC_HIP(hipGetLastError());
                 const uint64_t lines2 = static_cast<uint64_t>(dc.shape[0]) * dc.shape[1] * dc.shape[3];
