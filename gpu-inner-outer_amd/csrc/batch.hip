@@ -223,6 +223,7 @@ void gsc_batch::plan() {
         b.sat.probe_base = s.probe_base;
         // Written by chunk_flags_batch_kernel, which runs where frontier skipping does.
         b.sat.probe_or_bits = s.probe_or_used() ? s.probe_or_bits : nullptr;
+        b.sat.valid_bits = s.valid_bits;
         b.sat.rec = s.rec;
         b.sat.overflow = s.overflow;
         b.sat.box2 = s.box2;

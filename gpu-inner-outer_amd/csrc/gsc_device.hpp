@@ -25,11 +25,16 @@ constexpr double kEps = 1e-9;  // grid.hpp:186
 // (abstraction.hpp:32-38). Here the successor box is split: its θ and v ranges
 // depend only on (θ cell, ω input) and (v cell, a input) — the θ and v
 // trajectories of model.hpp:26-46 do not involve x or y — so they live in two
-// small tables (box2, box3). The per-pair word keeps the x and y ranges,
-// relative to the state's own cell:
-//   bits  0- 9  x offset  (box.min[0] - ix, two's complement, -512..511)
-//   bits 10-14  x extent  (box.max[0] - box.min[0], 0..31)
-//   bits 15-24  y offset
+// small tables (box2, box3). The per-pair word keeps the x and y ranges as
+// the cell where the box's middle x-row starts, relative to the state's own
+// cell q = ix + iy * n_x, and the two extents:
+//   bits  0-19  rel = (lo0 - ix) + (lo1 + ext1 / 2 - iy) * n_x, two's
+//               complement (-2^19 .. 2^19 - 1). q + rel is the middle row's
+//               first cell in the plane; lo0 = (q + rel) mod n_x and
+//               lo1 = (q + rel) / n_x - ext1 / 2, unique because lo0 lies in
+//               [0, n_x). The satisfaction probe reads that row as is.
+//   bits 20-24  31 - x extent (box.max[0] - box.min[0], 0..31): the probe's
+//               shift amount
 //   bits 25-29  y extent
 //   bit  30     dead: the box holds no valid cell, so the pair can never be
 //               satisfied (the reference's `valid_count == 0` test,
@@ -45,21 +50,31 @@ constexpr uint32_t kRecInvalid = 0xFFFFFFFFu;
 constexpr uint32_t kRecEscape = 0x80000000u;
 constexpr uint32_t kRecDead = 0x40000000u;
 constexpr uint32_t kRecIndexMask = 0x7FFFFFFFu;
-constexpr int kOffBits = 10;
-constexpr int kOffMin = -(1 << (kOffBits - 1));
-constexpr int kOffMax = (1 << (kOffBits - 1)) - 1;
+constexpr int kRelMin = -(1 << 19);
+constexpr int kRelMax = (1 << 19) - 1;
 constexpr uint32_t kExtMax = 31;
 // box2/box3 entry whose range is out of the state domain (pair invalid).
 constexpr uint32_t kBoxReject = 0xFFFFFFFFu;
 
-GSC_HD inline uint32_t rec_pack(int off0, uint32_t ext0, int off1, uint32_t ext1, bool dead) {
-    return (static_cast<uint32_t>(off0) & 0x3FFu) | (ext0 << 10) | ((static_cast<uint32_t>(off1) & 0x3FFu) << 15) |
-           (ext1 << 25) | (dead ? kRecDead : 0u);
+// The middle-row offset of a box with x / y offsets off0, off1 (lo - cell) and
+// y extent ext1, on a grid n_x cells wide.
+GSC_HD inline int64_t rec_rel_of(int off0, int off1, uint32_t ext1, uint32_t n0) {
+    return static_cast<int64_t>(off0) + (static_cast<int64_t>(off1) + static_cast<int64_t>(ext1 >> 1)) * n0;
 }
-GSC_HD inline int rec_off0(uint32_t r) { return static_cast<int32_t>(r << 22) >> 22; }
-GSC_HD inline uint32_t rec_ext0(uint32_t r) { return (r >> 10) & 31u; }
-GSC_HD inline int rec_off1(uint32_t r) { return static_cast<int32_t>(r << 7) >> 22; }
+GSC_HD inline uint32_t rec_pack(int rel, uint32_t ext0, uint32_t ext1, bool dead) {
+    return (static_cast<uint32_t>(rel) & 0xFFFFFu) | ((31u - ext0) << 20) | (ext1 << 25) | (dead ? kRecDead : 0u);
+}
+GSC_HD inline int rec_rel(uint32_t r) { return static_cast<int32_t>(r << 12) >> 12; }
+GSC_HD inline uint32_t rec_shift0(uint32_t r) { return (r >> 20) & 31u; }  // 31 - x extent
+GSC_HD inline uint32_t rec_ext0(uint32_t r) { return 31u - rec_shift0(r); }
 GSC_HD inline uint32_t rec_ext1(uint32_t r) { return (r >> 25) & 31u; }
+// The box's lower x and y cells, for the state at cell q of its plane.
+GSC_HD inline void rec_lo(uint32_t r, uint32_t q, uint32_t n0, uint32_t* lo0, uint32_t* lo1) {
+    const uint32_t t = q + static_cast<uint32_t>(rec_rel(r));  // the middle row's first cell, in [0, PQ)
+    const uint32_t ym = t / n0;
+    *lo0 = t - ym * n0;
+    *lo1 = ym - (rec_ext1(r) >> 1);
+}
 
 // Record placement. Within a (θ, v) plane (plane base = plane * I_pad * PQ,
 // PQ = cells per plane) the records come in blocks of kRecBlock consecutive

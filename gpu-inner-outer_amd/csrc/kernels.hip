@@ -76,6 +76,11 @@ __device__ __forceinline__ bool bits_any(uint32_t w0, uint32_t w1, uint32_t row,
     const uint32_t window = __builtin_amdgcn_alignbit(w1, w0, row & 31u);
     return (window << (31u - ext)) != 0;
 }
+// The same with the shift 31 - ext given (a record's rec_shift0).
+__device__ __forceinline__ bool bits_any_sh(uint32_t w0, uint32_t w1, uint32_t row, uint32_t sh) {
+    const uint32_t window = __builtin_amdgcn_alignbit(w1, w0, row & 31u);
+    return (window << sh) != 0;
+}
 
 __device__ inline bool loop_done(const IterCtl* ctl) {
     return __atomic_load_n(&ctl->done, __ATOMIC_RELAXED) != 0;
@@ -257,6 +262,13 @@ __global__ __launch_bounds__(256) void unreached_bits_kernel(uint32_t S, const u
                                                               unsigned long long* __restrict__ bits, const IterCtl* ctl) {
     if (ctl && loop_done(ctl)) return;
     unreached_bits_body(S, valid, reach, bits, blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void valid_bits_kernel(uint32_t S, const uint8_t* __restrict__ valid,
+                                                          unsigned long long* __restrict__ bits) {
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    const uint64_t b = __ballot(s < S && valid[s] != 0);
+    if (lane_id() == 0 && s < S) bits[s >> 6] = b;
 }
 
 // Pass 2..4: running sum along `axis` (1, 2 or 3); one thread per line (256 per
@@ -663,11 +675,16 @@ __global__ __launch_bounds__(256) void pair_rows_kernel(DevCase c, const uint2* 
                 const int32_t* const o = offs + (static_cast<size_t>(plane) * c.I + i) * 8;
                 const bool inv2 = b2.x > b2.y, inv3 = b3.x > b3.y;
                 bool fast = wd >= 2 && (!inv2 || wd == 2) && (!inv3 || wd == 3);
+                int64_t m[2] = {0, 0};  // largest |offset| per axis
                 for (int d = 0; d < 2; ++d) {
                     const int32_t lo = o[4 * d], hi = o[4 * d + 1];
-                    fast = fast && lo != kOffsetsUndecided && lo >= kOffMin && hi <= kOffMax && lo <= hi &&
-                           hi - lo <= static_cast<int32_t>(kExtMax);
+                    fast = fast && lo != kOffsetsUndecided && lo <= hi && hi - lo <= static_cast<int32_t>(kExtMax);
+                    m[d] = max(lo < 0 ? -static_cast<int64_t>(lo) : lo, hi < 0 ? -static_cast<int64_t>(hi) : hi);
                 }
+                // Every clamped box's middle-row offset fits the record: its x
+                // offset and its middle y offset (between its y offsets) are
+                // bounded by the unclamped ones.
+                fast = fast && m[0] + m[1] * c.shape[0] <= kRelMax;
                 const bool empty = inv3 || (inv2 && wd != 2);
                 kind = !fast ? kRowGeneral : constrained && !empty ? kRowFastChecked : kRowFast;
                 if (fast) {
@@ -869,7 +886,7 @@ struct LaneHull {
 // A lane's cell as the fast rows use it: a range clamped to the grid and taken
 // relative to the cell is clamp(i + off, 0, n - 1) - i = med3(off, -i, n - 1 - i).
 struct FastLane {
-    int32_t ix, iy, nix, tix, niy, tiy;
+    int32_t ix, iy, nix, tix, niy, tiy, n0;
     uint32_t fail_row_shift;
     const char* fail_bytes;
     bool state_ok;
@@ -884,6 +901,7 @@ __device__ __forceinline__ FastLane fast_lane(const DevCase& c, const uint32_t* 
     L.tix = static_cast<int32_t>(c.shape[0]) - 1 - L.ix;
     L.niy = -L.iy;
     L.tiy = static_cast<int32_t>(c.shape[1]) - 1 - L.iy;
+    L.n0 = static_cast<int32_t>(c.shape[0]);
     L.fail_row_shift = c.fail_shift + 2;
     L.fail_bytes = reinterpret_cast<const char*>(fail2d);
     L.state_ok = state_ok;
@@ -920,8 +938,10 @@ __device__ __forceinline__ uint32_t fast_record(const ConstPtr<int32_t> o, bool 
         H.y0 = min(H.y0, d1);
         H.y1 = max(H.y1, e1);
     }
-    return ok ? rec_pack(d0, static_cast<uint32_t>(e0 - d0), d1, static_cast<uint32_t>(e1 - d1), false)
-              : kRecInvalid;
+    const uint32_t ext1 = static_cast<uint32_t>(e1 - d1);
+    // 24-bit product: the row's offsets fit kRelMax (pair_rows_kernel).
+    const int rel = d0 + __mul24(d1 + static_cast<int32_t>(ext1 >> 1), L.n0);
+    return ok ? rec_pack(rel, static_cast<uint32_t>(e0 - d0), ext1, false) : kRecInvalid;
 }
 
 // One lane's record block (inputs i0 .. i0 + 7 of cell q) as two 16-byte stores.
@@ -1038,8 +1058,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void a
                 const int off0 = static_cast<int>(lo[0]) - static_cast<int>(ix);
                 const int off1 = static_cast<int>(lo[1]) - static_cast<int>(iy);
                 const uint32_t ext0 = hi[0] - lo[0], ext1 = hi[1] - lo[1];
-                const bool fits = regular & (off0 >= kOffMin) & (off0 <= kOffMax) & (off1 >= kOffMin) &
-                                  (off1 <= kOffMax) & (ext0 <= kExtMax) & (ext1 <= kExtMax);
+                const int64_t rel = rec_rel_of(off0, off1, ext1, c.shape[0]);
+                const bool fits = regular & (rel >= kRelMin) & (rel <= kRelMax) & (ext0 <= kExtMax) &
+                                  (ext1 <= kExtMax);
                 const bool packed = ok & fits;
                 H.any_valid |= ok;
                 H.unrepresentable |= ok & !fits;
@@ -1047,7 +1068,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void a
                 H.x1 = packed ? max(H.x1, off0 + static_cast<int>(ext0)) : H.x1;
                 H.y0 = packed ? min(H.y0, off1) : H.y0;
                 H.y1 = packed ? max(H.y1, off1 + static_cast<int>(ext1)) : H.y1;
-                uint32_t r = packed ? rec_pack(off0, ext0, off1, ext1, false) : kRecInvalid;
+                uint32_t r = packed ? rec_pack(static_cast<int>(rel), ext0, ext1, false) : kRecInvalid;
                 if (store && ok && !fits)
                     r = escape_record(kernarg_block<AbstractionArgs>(), lo[0], lo[1], lo[2], lo[3], hi[0], hi[1],
                                       hi[2], hi[3]);
@@ -1129,13 +1150,12 @@ __global__ __launch_bounds__(256) void export_kernel(DevCase c, const uint32_t* 
         out_valid[o] = 1;
         return;
     }
-    const uint32_t iy = q / c.shape[0];
-    const uint32_t ix = q - iy * c.shape[0];
     const uint32_t iz = plane % c.shape[2];
     const uint32_t iv = plane / c.shape[2];
     const uint2 b2 = box2[iz * c.n_w + i / c.n_a];
     const uint2 b3 = box3[iv * c.n_a + i % c.n_a];
-    const uint32_t lo0 = ix + rec_off0(r), lo1 = iy + rec_off1(r);
+    uint32_t lo0, lo1;
+    rec_lo(r, q, c.shape[0], &lo0, &lo1);
     out_min[o] = lo0 * c.stride[0] + lo1 * c.stride[1] + b2.x * c.stride[2] + b3.x * c.stride[3];
     out_max[o] = (lo0 + rec_ext0(r)) * c.stride[0] + (lo1 + rec_ext1(r)) * c.stride[1] + b2.y * c.stride[2] +
                  b3.y * c.stride[3];
@@ -1149,8 +1169,8 @@ __global__ __launch_bounds__(256) void export_kernel(DevCase c, const uint32_t* 
 // sat = !dead && count(unreached valid cells in the box) == 0, which equals the
 // reference's `valid && cntV > 0 && cntR == cntV` because reachable ⊆ valid
 // makes cntV - cntR the count of valid, unreached cells.
-__device__ inline bool count_test(const SatisfyArgs& A, const uint32_t* PU, uint32_t r, uint32_t ix, uint32_t iy,
-                                  uint2 b2, uint2 b3, uint32_t* rec_slot) {
+__device__ inline bool count_test(const SatisfyArgs& A, const uint32_t* PU, uint32_t r, uint32_t q, uint2 b2, uint2 b3,
+                                  uint32_t* rec_slot) {
     const DevCase& c = A.c;
     const int wd = c.wrap_dim;
     if (r & kRecEscape) {
@@ -1162,9 +1182,8 @@ __device__ inline bool count_test(const SatisfyArgs& A, const uint32_t* PU, uint
         return box_count_i64(PU, c.pstride, wd, c.shape[wd], lo, hi) == 0;
     }
     uint32_t lo[4], hi[4];
-    lo[0] = ix + rec_off0(r);
+    rec_lo(r, q, c.shape[0], &lo[0], &lo[1]);
     hi[0] = lo[0] + rec_ext0(r);
-    lo[1] = iy + rec_off1(r);
     hi[1] = lo[1] + rec_ext1(r);
     lo[2] = b2.x;
     hi[2] = b2.y;
@@ -1180,36 +1199,106 @@ __device__ inline bool count_test(const SatisfyArgs& A, const uint32_t* PU, uint
     return true;
 }
 
-// The full test of one pair that the row probe could not rule out: two more
-// rows of the unreached bitmask — the box's first and last y rows at its
-// x-range and lower θ, v, where the unreached cells left in a box tend to sit
-// — and, if neither holds one, the 16-corner count (count_test). `counted`
-// counts the pairs that reach the count.
+// The box's θ (or v) range as a cell count, wrapping past the axis end (a
+// compact record's range wraps only on the wrap axis).
+__device__ __forceinline__ uint32_t range_cells(uint2 b, uint32_t n) {
+    return (b.y >= b.x ? b.y - b.x : b.y + n - b.x) + 1u;
+}
+
+// Whether any of bits [row, row + 31 - sh] is set in the rows a + y * st1,
+// y = 0 .. e1 (e1 < 8): the eight row loads are issued together (a short box
+// re-reads its last row).
+__device__ __forceinline__ bool rows_any_8(const uint32_t* __restrict__ bm, uint32_t a, uint32_t st1, uint32_t e1,
+                                           uint32_t sh) {
+    uint32_t w0[8], w1[8];
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) {
+        const uint32_t rw = a + min(j, e1) * st1;
+        w0[j] = bm[rw >> 5];
+        w1[j] = bm[(rw >> 5) + 1];
+    }
+    bool any = false;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) any |= bits_any_sh(w0[j], w1[j], a + min(j, e1) * st1, sh);
+    return any;
+}
+
+// The full test of one pair that the row probe could not rule out (or of an
+// escaped pair). Almost every such pair turns out satisfied, and a satisfied
+// pair must be shown free of unreached valid cells everywhere in its box, so
+// the test scans the whole box where that is a few rows: the record, the
+// probe base and the θ / v ranges are loaded together, then every x-row of
+// the box (up to two θ / v slices of the probe bitmask, eight rows each, a
+// round of loads per slice) from the cache-resident bitmask, and finally one row of
+// the valid bitmask, which shows a valid cell in the box in almost every case.
+// Other boxes take two more rows of the probe bitmask (the first and last y
+// rows, where the unreached cells left in a box tend to sit) and the
+// 16-corner count (count_test). `counted` counts the pairs that reach a
+// prefix-table count.
 __device__ __forceinline__ bool pair_satisfied(const SatisfyArgs& A, const uint32_t* PU, uint32_t* rec_plane,
                                                const uint32_t* pb_plane, uint32_t plane, uint32_t qs, uint32_t i,
                                                uint32_t& counted) {
     const DevCase& c = A.c;
     const uint32_t st1 = c.stride[1];
-    const uint32_t sy = qs / c.shape[0];
-    const uint32_t sx = qs - sy * c.shape[0];
     uint32_t* const slot = rec_plane + rec_offset(c.PQ, qs, i);
-    const uint32_t rj = *slot;
-    if (!(rj & kRecEscape)) {
-        // The bitmask of the block probe (the OR bitmask for a box spanning two
-        // θ and two v cells, where it is built): the pair's probe base.
-        const uint32_t* const bm = reinterpret_cast<const uint32_t*>(A.unreached_bits);
-        const uint32_t ra = sx + sy * st1 + rec_off0(rj) +
-                            static_cast<uint32_t>(__mul24(rec_off1(rj), static_cast<int>(st1))) + pb_plane[i];
-        const uint32_t rb = ra + static_cast<uint32_t>(__mul24(static_cast<int>(rec_ext1(rj)), static_cast<int>(st1)));
-        const uint32_t e = rec_ext0(rj);
-        if (bits_any(bm[ra >> 5], bm[(ra >> 5) + 1], ra, e) || bits_any(bm[rb >> 5], bm[(rb >> 5) + 1], rb, e))
-            return false;
-    }
     const uint32_t iw = i / c.n_a;
+    const uint32_t rj = *slot;
+    const uint32_t pbi = pb_plane[i];
     const uint2 b2 = A.box2[(plane % c.shape[2]) * c.n_w + iw];
     const uint2 b3 = A.box3[(plane / c.shape[2]) * c.n_a + (i - iw * c.n_a)];
+    if (!(rj & kRecEscape)) {
+        const uint32_t* const bm = reinterpret_cast<const uint32_t*>(A.unreached_bits);
+        const uint32_t e1 = rec_ext1(rj), sh = rec_shift0(rj);
+        const uint32_t mid = qs + static_cast<uint32_t>(rec_rel(rj));  // the middle row's first cell
+        const uint32_t first = mid - static_cast<uint32_t>(__mul24(static_cast<int>(e1 >> 1), static_cast<int>(st1)));
+        // The slices of the probe bitmask the box needs: with the OR bitmask
+        // (pbi past its start) a slice at θ cell z, v cell w covers z, z + 1,
+        // w, w + 1, so a range of two cells takes one slice and of three two
+        // (the second starting one cell up, wrapping like the OR bitmask's
+        // neighbours); the plain bitmask takes one slice per cell.
+        const uint32_t n2 = c.shape[2], n3 = c.shape[3];
+        const uint32_t nz = range_cells(b2, n2), nw = range_cells(b3, n3);
+        const bool orb = pbi >= probe_or_bit0(c);
+        const uint32_t sz = orb ? nz - 1u : nz, sw = orb ? nw - 1u : nw;
+        if (e1 < 8u && sz * sw <= 2u) {
+            const uint32_t pitch = orb ? probe_or_pitch(c) : c.PQ;
+            // The second slice: one θ cell up (sz = 2) or one v cell up (sw = 2).
+            const uint32_t z1 = b2.x + 1u == n2 ? 0u : b2.x + 1u, w1 = b3.x + 1u == n3 ? 0u : b3.x + 1u;
+            const uint32_t p0 = b2.x + b3.x * n2;
+            const uint32_t p1 = sz == 2u ? z1 + b3.x * n2 : sw == 2u ? b2.x + w1 * n2 : p0;
+            const uint32_t a = first + pbi;
+            if (rows_any_8(bm, a, st1, e1, sh)) return false;
+            if (p1 != p0 && rows_any_8(bm, a + (p1 - p0) * pitch, st1, e1, sh)) return false;
+            // No unreached valid cell in the box: satisfied unless it holds no
+            // valid cell at all. One row of the valid bitmask (the middle row at
+            // the lower θ and v) almost always shows one; otherwise the count
+            // decides, and a box with none is memoised dead.
+            const uint32_t* const vb = reinterpret_cast<const uint32_t*>(A.valid_bits);
+            const uint32_t vr = p0 * c.PQ + mid;
+            if (bits_any_sh(vb[vr >> 5], vb[(vr >> 5) + 1], vr, sh)) return true;
+            uint32_t lo[4], hi[4];
+            rec_lo(rj, qs, c.shape[0], &lo[0], &lo[1]);
+            hi[0] = lo[0] + rec_ext0(rj);
+            hi[1] = lo[1] + e1;
+            lo[2] = b2.x;
+            hi[2] = b2.y;
+            lo[3] = b3.x;
+            hi[3] = b3.y;
+            ++counted;
+            if (box_count_wrap_u32(A.prefix_valid, c.pstride, c.wrap_dim, c.shape[c.wrap_dim], lo, hi) == 0) {
+                *slot = rj | kRecDead;
+                return false;
+            }
+            return true;
+        }
+        // The first and last y rows at the x-range and lower θ, v.
+        const uint32_t ra = first + pbi;
+        const uint32_t rb = ra + static_cast<uint32_t>(__mul24(static_cast<int>(e1), static_cast<int>(st1)));
+        if (bits_any_sh(bm[ra >> 5], bm[(ra >> 5) + 1], ra, sh) || bits_any_sh(bm[rb >> 5], bm[(rb >> 5) + 1], rb, sh))
+            return false;
+    }
     ++counted;
-    return count_test(A, PU, rj, sx, sy, b2, b3, slot);
+    return count_test(A, PU, rj, qs, b2, b3, slot);
 }
 
 // Totals and the stop test of one iteration (kernels.cu:722-766), by one wave.

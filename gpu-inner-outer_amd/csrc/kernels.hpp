@@ -113,6 +113,9 @@ struct SatisfyArgs {
     // flag pass (chunk_flags_body); null where that pass does not run, and then
     // no probe base points into it.
     const unsigned long long* probe_or_bits;
+    // Bitmask of the valid states (flat state order, one word past the last
+    // state's), written once at set-up.
+    const unsigned long long* valid_bits;
     uint32_t* rec;
     const OverflowEntry* overflow;
     const uint2* box2;
@@ -189,12 +192,16 @@ __global__ void coarse_kernel(DevCase c, const uint32_t* P0, const uint32_t* P1,
 __global__ void coarse_batch_kernel(const BatchInstance* B, const uint32_t* starts, uint32_t n);
 __global__ void unreached_bits_kernel(uint32_t S, const uint8_t* valid, const uint8_t* reach, unsigned long long* bits,
                                       const IterCtl* ctl);
+// Bitmask of the valid states (bit s of word s / 64), once per set-up.
+__global__ void valid_bits_kernel(uint32_t S, const uint8_t* valid, unsigned long long* bits);
 // Row probes (see satisfy_kernel): a row of x cells at one y, θ and v, as bits
 // [row, row + extent] of a bitmask. unreached_bits (flat state order, one word
 // past the last state's) is followed in the same allocation by probe_or_bits
 // (n_tasks + 1 words), so both are read from one base: a row starts at bit
 // x + y*n_x + plane*PQ of the first, or at bit probe_or_bit0(c) +
 // plane*probe_or_pitch(c) + x + y*n_x, where each plane starts on a word.
+// The valid bitmask (probe_bits_words words, flat order like the first)
+// follows the OR bitmask in the same allocation.
 __host__ __device__ inline uint32_t probe_or_pitch(const DevCase& c) { return c.chunks_per_plane * 64u; }
 __host__ __device__ inline uint32_t probe_bits_words(const DevCase& c) { return (c.S + 63u) / 64u + 1u; }
 __host__ __device__ inline uint32_t probe_or_bit0(const DevCase& c) { return probe_bits_words(c) * 64u; }

@@ -239,9 +239,11 @@ void gsc_solver::setup_launch() {
     probe_base = alloc<uint32_t>(n_planes * dc.I_pad);
     row_kind = alloc<uint32_t>(n_planes * dc.I_pad);
     // The plain probe bitmask (+1 word: the row probe reads one word past),
-    // then the plane-aligned OR bitmask (one word per chunk, +1).
-    ubits = alloc<unsigned long long>(static_cast<uint64_t>(probe_bits_words(dc)) + dc.n_tasks + 1);
+    // then the plane-aligned OR bitmask (one word per chunk, +1), then the
+    // valid bitmask (as the plain one).
+    ubits = alloc<unsigned long long>(2ull * probe_bits_words(dc) + dc.n_tasks + 1);
     probe_or_bits = ubits + probe_bits_words(dc);
+    valid_bits = probe_or_bits + dc.n_tasks + 1;
     overflow_capacity = opts.recompute_abstraction ? 1u : 1u << 16;
     overflow = alloc<OverflowEntry>(overflow_capacity);
     overflow_count = alloc<uint32_t>(1);
@@ -314,8 +316,12 @@ void gsc_solver::setup_launch() {
     GSC_HIP(hipGetLastError());
     unsigned long long* counts = reinterpret_cast<unsigned long long*>(host_words + kWordMaskCounts);
     GSC_HIP(hipMemcpyAsync(counts, mask_counts, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
-    // The valid prefix does not wait for the check: a failed check discards it.
+    // The valid prefix and bitmask do not wait for the check: a failed check
+    // discards them.
     build_prefix(valid, nullptr, prefix_valid, prefix_valid, nullptr);
+    hipLaunchKernelGGL(valid_bits_kernel, dim3(blocks_for(S, 256)), dim3(256), 0, stream, static_cast<uint32_t>(S),
+                       valid, valid_bits);
+    GSC_HIP(hipGetLastError());
     build_coarse(prefix_valid, prefix_valid, coarse_valid, coarse_valid, nullptr);
     setup_t0 = t0;
 }
@@ -346,7 +352,7 @@ uint64_t gsc_solver::device_bytes(const gsc_case_shape& sh, const gsc_case_desc&
     b += r(planes * (((n0 * n1 + 63) / 64 + 15) / 16 * 16));
     b += 3 * r(4 * ((n0 + kCoarse - 1) / kCoarse + 1) * ((n1 + kCoarse - 1) / kCoarse + 1) * (n2 + 1) * (n3 + 1));
     b += r(4 * planes * I_pad * n0 * n1) + 2 * r(4 * planes * I_pad);  // records, probe bases, row kinds
-    b += r(8 * ((S + 63) / 64 + 1 + planes * ((n0 * n1 + 63) / 64) + 1));  // both probe bitmasks
+    b += r(8 * (2 * ((S + 63) / 64 + 1) + planes * ((n0 * n1 + 63) / 64) + 1));  // the three bitmasks
     b += r(sizeof(OverflowEntry) * (1u << 16)) + r(4) + r(16) + r(sizeof(IterCtl)) + r(sizeof(IterRecord) * kRing);
     b += r(16 * n2 * nw * K) + r(8 * n3 * na * K) + r(8 * n2 * nw) + r(8 * n3 * na) + r(16 * n3);
     b += r(4 * (n0 + 1) * (n1 + 1)) + r(16 * planes * I * K) + r(32 * planes * I) + 2 * r(64 * 64);
@@ -546,6 +552,7 @@ void gsc_solver::solve() {
     A.unreached_bits = ubits;
     A.probe_base = probe_base;
     A.probe_or_bits = probe_or_used() ? probe_or_bits : nullptr;
+    A.valid_bits = valid_bits;
     const bool chunk_flags_used = !opts.sweep_all_pairs && dc.hull_enabled;
     A.rec = rec;
     A.overflow = overflow;

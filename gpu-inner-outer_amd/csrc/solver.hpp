@@ -250,6 +250,7 @@ struct gsc_solver {
     int32_t* cell_offs = nullptr;  // [plane][input][8], see increment_sums_kernel
     unsigned long long* ubits = nullptr;
     unsigned long long* probe_or_bits = nullptr;  // [n_tasks + 1], plane-aligned (see chunk_flags_body)
+    unsigned long long* valid_bits = nullptr;     // [S / 64 + 1], the valid states (set-up)
     uint32_t* probe_base = nullptr;  // [plane][I_pad], see pair_rows_kernel
     uint32_t* row_kind = nullptr;    // [plane][I_pad], see pair_rows_kernel
     uint4* row_box = nullptr;        // [plane][input], see pair_rows_kernel
