@@ -652,7 +652,8 @@ __global__ __launch_bounds__(256) void pair_rows_kernel(DevCase c, const uint2* 
                                                          uint32_t* __restrict__ probe_base,
                                                          uint32_t* __restrict__ row_kind, uint4* __restrict__ row_box,
                                                          FastRow* __restrict__ fast_offs,
-                                                         uint32_t* __restrict__ plane_mixed) {
+                                                         uint32_t* __restrict__ plane_mixed,
+                                                         uint32_t* __restrict__ ext1_max) {
     const uint32_t plane = blockIdx.x * 4u + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (plane >= c.shape[2] * c.shape[3]) return;
     const uint32_t lane = lane_id();
@@ -660,6 +661,7 @@ __global__ __launch_bounds__(256) void pair_rows_kernel(DevCase c, const uint2* 
     const int wd = c.wrap_dim;
     const bool constrained = c.constraint_type != GSC_CONSTRAINT_NONE;
     bool mixed = false;
+    uint32_t ext1 = 0;  // the fast rows' largest unclamped y extent
     for (uint32_t i = lane; i < c.I_pad; i += 64) {
         uint32_t v = 0, kind = kRowInvalid;
         FastRow f{{0, 0}, {0, 0}, {kFastNoCell, kFastNoCell}, {0, 0}};
@@ -688,6 +690,7 @@ __global__ __launch_bounds__(256) void pair_rows_kernel(DevCase c, const uint2* 
                 const bool empty = inv3 || (inv2 && wd != 2);
                 kind = !fast ? kRowGeneral : constrained && !empty ? kRowFastChecked : kRowFast;
                 if (fast) {
+                    ext1 = max(ext1, static_cast<uint32_t>(o[5] - o[4]));
                     // Cells i with rej_lo <= i <= rej_hi, within [0, n - 1].
                     bool any_cell = true;
                     for (int d = 0; d < 2; ++d) {
@@ -711,6 +714,10 @@ __global__ __launch_bounds__(256) void pair_rows_kernel(DevCase c, const uint2* 
     }
     const bool any = __ballot(mixed) != 0;
     if (lane == 0) plane_mixed[plane] = any ? 1u : 0u;
+    // Clamping to the grid only shortens an extent, so this bounds every
+    // compact box of the plane's fast rows.
+    for (int off = 32; off > 0; off >>= 1) ext1 = max(ext1, static_cast<uint32_t>(__shfl_xor(ext1, off, 64)));
+    if (lane == 0 && ext1 != 0) atomicMax(ext1_max, ext1);
 }
 
 // The kernel's own argument block, addressed as ordinary (global) memory, so
@@ -881,6 +888,7 @@ __device__ __forceinline__ int32_t med3_i32(int32_t a, int32_t b, int32_t c) {
 struct LaneHull {
     int x0, x1, y0, y1;
     bool any_valid, unrepresentable;
+    int e1;  // largest y extent of the lane's compact records
 };
 
 // A lane's cell as the fast rows use it: a range clamped to the grid and taken
@@ -996,7 +1004,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void a
     uint32_t* const rec_plane = A.rec + static_cast<uint64_t>(plane) * c.I_pad * c.PQ;
     const bool store = A.rec != nullptr;  // false: state hulls only
     // Running hull of the state's valid boxes (see gsc_device.hpp).
-    LaneHull H{kHullMaxOffset + 1, -kHullMaxOffset - 1, kHullMaxOffset + 1, -kHullMaxOffset - 1, false, false};
+    LaneHull H{kHullMaxOffset + 1, -kHullMaxOffset - 1, kHullMaxOffset + 1, -kHullMaxOffset - 1, false, false, 0};
     // Per-input row data is wave-uniform: scalar loads through the constant
     // address space (a plain load could alias the record stores below, and
     // would be a vector load waited for on every input).
@@ -1068,6 +1076,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void a
                 H.x1 = packed ? max(H.x1, off0 + static_cast<int>(ext0)) : H.x1;
                 H.y0 = packed ? min(H.y0, off1) : H.y0;
                 H.y1 = packed ? max(H.y1, off1 + static_cast<int>(ext1)) : H.y1;
+                H.e1 = packed ? max(H.e1, static_cast<int>(ext1)) : H.e1;
                 uint32_t r = packed ? rec_pack(static_cast<int>(rel), ext0, ext1, false) : kRecInvalid;
                 if (store && ok && !fits)
                     r = escape_record(kernarg_block<AbstractionArgs>(), lo[0], lo[1], lo[2], lo[3], hi[0], hi[1],
@@ -1079,6 +1088,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void a
     }
     const int hx0 = H.x0, hx1 = H.x1, hy0 = H.y0, hy1 = H.y1;
     const bool any_valid = H.any_valid, unrepresentable = H.unrepresentable;
+    if (!kFast && A.ext1_max) {
+        // The tallest compact box of a general row (pair_rows_kernel bounds the
+        // fast rows'), for the satisfaction's row scans.
+        uint32_t e = live ? static_cast<uint32_t>(H.e1) : 0u;
+        for (int off = 32; off > 0; off >>= 1) e = max(e, static_cast<uint32_t>(__shfl_xor(e, off, 64)));
+        if (lane == 0 && e != 0) atomicMax(A.ext1_max, e);
+    }
     if (live && A.hull) {
         uint32_t h;
         if (!any_valid)
@@ -1163,12 +1179,14 @@ __global__ __launch_bounds__(256) void export_kernel(DevCase c, const uint32_t* 
 }
 
 // ---------------------------------------------------------------------------
-// Pair satisfaction, the rare slow part: a full prefix count of the box of one
-// pair that the bitmask probe could not rule out (or an escaped pair).
+// Pair satisfaction by prefix counts, for the boxes the row scans do not
+// cover: an escaped pair, or a compact box of more than sixteen rows or more
+// than two bitmask slices (only where the unreached prefix table is built).
 //
 // sat = !dead && count(unreached valid cells in the box) == 0, which equals the
 // reference's `valid && cntV > 0 && cntR == cntV` because reachable ⊆ valid
-// makes cntV - cntR the count of valid, unreached cells.
+// makes cntV - cntR the count of valid, unreached cells. An overflow entry
+// settled `dead` when it was written; a compact record learns it here.
 __device__ inline bool count_test(const SatisfyArgs& A, const uint32_t* PU, uint32_t r, uint32_t q, uint2 b2, uint2 b3,
                                   uint32_t* rec_slot) {
     const DevCase& c = A.c;
@@ -1223,18 +1241,30 @@ __device__ __forceinline__ bool rows_any_8(const uint32_t* __restrict__ bm, uint
     return any;
 }
 
+// The same for a box of up to sixteen rows (e1 < 16): a second round of eight
+// for the rows past the eighth.
+__device__ __forceinline__ bool rows_any_16(const uint32_t* __restrict__ bm, uint32_t a, uint32_t st1, uint32_t e1,
+                                            uint32_t sh) {
+    if (rows_any_8(bm, a, st1, min(e1, 7u), sh)) return true;
+    return e1 >= 8u && rows_any_8(bm, a + 8u * st1, st1, e1 - 8u, sh);
+}
+
 // The full test of one pair that the row probe could not rule out (or of an
 // escaped pair). Almost every such pair turns out satisfied, and a satisfied
 // pair must be shown free of unreached valid cells everywhere in its box, so
 // the test scans the whole box where that is a few rows: the record, the
 // probe base and the θ / v ranges are loaded together, then every x-row of
-// the box (up to two θ / v slices of the probe bitmask, eight rows each, a
-// round of loads per slice) from the cache-resident bitmask, and finally one row of
+// the box (up to two θ / v slices of the probe bitmask, up to sixteen rows
+// each, a round of eight row loads at a time) from the cache-resident
+// bitmask, and finally one row of
 // the valid bitmask, which shows a valid cell in the box in almost every case.
 // Other boxes take two more rows of the probe bitmask (the first and last y
 // rows, where the unreached cells left in a box tend to sit) and the
-// 16-corner count (count_test). `counted` counts the pairs that reach a
+// 16-corner counts (count_test). With kBits (frontier skipping on bitmasks,
+// where no unreached prefix table is built) the solver guarantees that every
+// box is compact and fits the scan. `counted` counts the pairs that reach a
 // prefix-table count.
+template <bool kBits>
 __device__ __forceinline__ bool pair_satisfied(const SatisfyArgs& A, const uint32_t* PU, uint32_t* rec_plane,
                                                const uint32_t* pb_plane, uint32_t plane, uint32_t qs, uint32_t i,
                                                uint32_t& counted) {
@@ -1260,15 +1290,15 @@ __device__ __forceinline__ bool pair_satisfied(const SatisfyArgs& A, const uint3
         const uint32_t nz = range_cells(b2, n2), nw = range_cells(b3, n3);
         const bool orb = pbi >= probe_or_bit0(c);
         const uint32_t sz = orb ? nz - 1u : nz, sw = orb ? nw - 1u : nw;
-        if (e1 < 8u && sz * sw <= 2u) {
+        if (kBits || (e1 < 16u && sz * sw <= 2u)) {
             const uint32_t pitch = orb ? probe_or_pitch(c) : c.PQ;
             // The second slice: one θ cell up (sz = 2) or one v cell up (sw = 2).
             const uint32_t z1 = b2.x + 1u == n2 ? 0u : b2.x + 1u, w1 = b3.x + 1u == n3 ? 0u : b3.x + 1u;
             const uint32_t p0 = b2.x + b3.x * n2;
             const uint32_t p1 = sz == 2u ? z1 + b3.x * n2 : sw == 2u ? b2.x + w1 * n2 : p0;
             const uint32_t a = first + pbi;
-            if (rows_any_8(bm, a, st1, e1, sh)) return false;
-            if (p1 != p0 && rows_any_8(bm, a + (p1 - p0) * pitch, st1, e1, sh)) return false;
+            if (rows_any_16(bm, a, st1, e1, sh)) return false;
+            if (p1 != p0 && rows_any_16(bm, a + (p1 - p0) * pitch, st1, e1, sh)) return false;
             // No unreached valid cell in the box: satisfied unless it holds no
             // valid cell at all. One row of the valid bitmask (the middle row at
             // the lower θ and v) almost always shows one; otherwise the count
@@ -1440,7 +1470,7 @@ __device__ __forceinline__ bool chunk_changed(const SatisfyArgs& A, uint4 ch, ui
 // any state.
 // kSweep: the sweep_all_pairs schedule, where each lane counts its own pairs
 // in input order instead of through the count queue.
-template <uint32_t WG, uint32_t IB, bool kSweep>
+template <uint32_t WG, uint32_t IB, bool kSweep, bool kBits>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) void satisfy_kernel(SatisfyArgs A,
                                                                                               uint32_t iter) {
     const uint32_t block = blockIdx.x;
@@ -1461,6 +1491,30 @@ __device__ __forceinline__ uint64_t bits64_at(const unsigned long long* __restri
     return sh ? (lo >> sh) | (static_cast<uint64_t>(b[w + 1]) << (64u - sh)) : lo;
 }
 
+// Word j of plane `plane` of the OR bitmask: the chunk's 64 cells of the flat
+// bitmask U ORed with the same cells one θ plane up, one v plane up, and both
+// (the next θ (v) plane wrapping to the first when θ (v) is the wrap axis,
+// absent past the last otherwise); cells past the plane's end read 0.
+__device__ __forceinline__ uint64_t or_probe_word(const DevCase& c, const unsigned long long* __restrict__ U,
+                                                  uint32_t plane, uint32_t j) {
+    const uint32_t q0 = j * 64u;
+    const uint32_t n2 = c.shape[2];
+    const uint32_t z = plane % n2, w = plane / n2;
+    const bool zl = z + 1u < n2, wl = w + 1u < c.shape[3];
+    const bool has_z = zl || c.wrap_dim == 2, has_w = wl || c.wrap_dim == 3;
+    const uint32_t pz = zl ? plane + 1u : plane - z;
+    const uint32_t pw = wl ? plane + n2 : plane - w * n2;
+    const uint32_t pzw = wl ? pz + n2 : pz - w * n2;
+    const uint32_t PQ = c.PQ;
+    uint64_t m = bits64_at(U, plane * PQ + q0);
+    if (has_z) m |= bits64_at(U, pz * PQ + q0);
+    if (has_w) m |= bits64_at(U, pw * PQ + q0);
+    if (has_z && has_w) m |= bits64_at(U, pzw * PQ + q0);
+    const uint32_t left = PQ - q0;  // cells of the plane from q0 on
+    if (left < 64u) m &= (1ull << left) - 1ull;
+    return m;
+}
+
 // Chunk k's flag, and (where the OR bitmask is used) its word of the OR
 // bitmask: the chunk's 64 cells of valid, unreached states ORed with the same
 // cells one θ plane up, one v plane up, and both — the next θ (v) plane wrapping
@@ -1475,31 +1529,271 @@ __device__ __forceinline__ void chunk_flags_body(const SatisfyArgs& A, uint8_t* 
     const uint32_t* const CU = A.coarse_unreached[iter & 1u];
     const uint32_t* const Cprev = iter == 1 ? A.coarse_valid : A.coarse_unreached[(iter - 1u) & 1u];
     flags[plane * chunk_flag_pitch(c) + j] = chunk_changed(A, A.chunk_hull[k], plane, CU, Cprev) ? 1u : 0u;
-    if (A.probe_or_bits) {
-        const uint32_t q0 = j * 64u;
-        const uint32_t n2 = c.shape[2];
-        const uint32_t z = plane % n2, w = plane / n2;
-        const bool zl = z + 1u < n2, wl = w + 1u < c.shape[3];
-        const bool has_z = zl || c.wrap_dim == 2, has_w = wl || c.wrap_dim == 3;
-        const uint32_t pz = zl ? plane + 1u : plane - z;
-        const uint32_t pw = wl ? plane + n2 : plane - w * n2;
-        const uint32_t pzw = wl ? pz + n2 : pz - w * n2;
-        const unsigned long long* const U = A.unreached_bits;
-        const uint32_t PQ = c.PQ;
-        uint64_t m = bits64_at(U, plane * PQ + q0);
-        if (has_z) m |= bits64_at(U, pz * PQ + q0);
-        if (has_w) m |= bits64_at(U, pw * PQ + q0);
-        if (has_z && has_w) m |= bits64_at(U, pzw * PQ + q0);
-        const uint32_t left = PQ - q0;  // cells of the plane from q0 on
-        if (left < 64u) m &= (1ull << left) - 1ull;
-        const_cast<unsigned long long*>(A.probe_or_bits)[k] = m;
-    }
+    if (A.probe_or_bits) const_cast<unsigned long long*>(A.probe_or_bits)[k] = or_probe_word(c, A.unreached_bits, plane, j);
 }
 
 __global__ __launch_bounds__(256) void chunk_flags_kernel(SatisfyArgs A, uint8_t* __restrict__ flags,
                                                           uint32_t iter) {
     if (loop_done(A.ctl)) return;
     chunk_flags_body(A, flags, iter, blockIdx.x * 256u + threadIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Frontier skipping from bitmasks. A state tested and not satisfied at
+// iteration k - 1 gets the same answer at k unless a cell of its hull turned
+// from unreached to reached in between (see satisfy_kernel). Here that test is
+// made for all states of a (θ, v) plane at once, on bitmasks:
+//   N(p')  = prev & ~cur over plane p': the cells reached since the previous
+//            iteration (before the first: every valid cell that is not
+//            unreached, i.e. the seeds);
+//   D(p)   = OR of N(p') over the planes p' of p's θ and v hull ranges
+//            (hull2, hull3, the unions over the inputs of the boxes' ranges);
+//   E(p)   = D(p) dilated by the plane's hull rectangle: bit (x, y) is set when
+//            D(p) has a bit at (x + dx, y + dy) for some offsets dx, dy inside
+//            the union of the plane's state hulls.
+// Every box of a state lies inside its hull, so a state whose E bit is clear
+// saw no change in any of its boxes and is skipped; E is a superset of the
+// per-state hull test, exact for every state whose hull is the plane's union
+// (all interior cells of a plane whose rows are fast rows). The satisfaction
+// reads E as one word per 64-cell chunk: a chunk whose word is 0 has nothing
+// to test. Set bits are kept only for states with a valid pair (keep), and
+// states whose hull is not representable are always tested (always).
+
+// Once per abstraction, one workgroup per plane: the keep and always words of
+// the plane's chunks, and the union of its representable state hulls
+// (x0 > x1 when it has none).
+__global__ __launch_bounds__(256) void frontier_setup_kernel(DevCase c, const uint32_t* __restrict__ hull,
+                                                             unsigned long long* __restrict__ keep,
+                                                             unsigned long long* __restrict__ always,
+                                                             int4* __restrict__ plane_rect) {
+    __shared__ int s_rect[4][4];
+    const uint32_t plane = blockIdx.x;
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    const uint32_t cpp = c.chunks_per_plane;
+    int x0 = kHullMaxOffset + 1, x1 = -kHullMaxOffset - 1, y0 = kHullMaxOffset + 1, y1 = -kHullMaxOffset - 1;
+    for (uint32_t j = wave; j < cpp; j += 4u) {
+        const uint32_t q = j * 64u + lane;
+        const uint32_t h = q < c.PQ ? hull[static_cast<size_t>(plane) * c.PQ + q] : kHullNone;
+        const uint64_t kb = __ballot(h != kHullNone), ab = __ballot(h == kHullAll);
+        if (lane == 0) {
+            keep[static_cast<size_t>(plane) * cpp + j] = kb;
+            always[static_cast<size_t>(plane) * cpp + j] = ab;
+        }
+        if (h != kHullNone && h != kHullAll) {
+            x0 = min(x0, hull_byte(h, 0));
+            x1 = max(x1, hull_byte(h, 1));
+            y0 = min(y0, hull_byte(h, 2));
+            y1 = max(y1, hull_byte(h, 3));
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        x0 = min(x0, __shfl_xor(x0, off, 64));
+        x1 = max(x1, __shfl_xor(x1, off, 64));
+        y0 = min(y0, __shfl_xor(y0, off, 64));
+        y1 = max(y1, __shfl_xor(y1, off, 64));
+    }
+    if (lane == 0) {
+        s_rect[wave][0] = x0;
+        s_rect[wave][1] = x1;
+        s_rect[wave][2] = y0;
+        s_rect[wave][3] = y1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) {
+            s_rect[0][0] = min(s_rect[0][0], s_rect[w][0]);
+            s_rect[0][1] = max(s_rect[0][1], s_rect[w][1]);
+            s_rect[0][2] = min(s_rect[0][2], s_rect[w][2]);
+            s_rect[0][3] = max(s_rect[0][3], s_rect[w][3]);
+        }
+        plane_rect[plane] = make_int4(s_rect[0][0], s_rect[0][1], s_rect[0][2], s_rect[0][3]);
+    }
+}
+
+// Eight states per thread: their valid and reach bytes as one 8-byte load
+// each, the previous unreached byte beside them, and one byte of each bitmask
+// written (bit s of a bitmask is bit s % 8 of its byte s / 8).
+__global__ __launch_bounds__(256) void frontier_bits_kernel(uint32_t S, const uint8_t* __restrict__ valid,
+                                                            const uint8_t* __restrict__ reach,
+                                                            unsigned long long* __restrict__ cur,
+                                                            unsigned long long* __restrict__ fresh,
+                                                            const unsigned long long* __restrict__ valid_bits,
+                                                            uint32_t iter, const IterCtl* ctl) {
+    if (loop_done(ctl)) return;
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t s0 = t * 8u;
+    if (s0 >= S) return;
+    uint8_t* const cb = reinterpret_cast<uint8_t*>(cur);
+    const uint8_t old = iter == 1 ? reinterpret_cast<const uint8_t*>(valid_bits)[t] : cb[t];
+    uint32_t b = 0;
+    if (s0 + 8u <= S) {
+        const uint2 v = *reinterpret_cast<const uint2*>(valid + s0);
+        const uint2 r = *reinterpret_cast<const uint2*>(reach + s0);
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            b |= static_cast<uint32_t>(((v.x >> (8 * j)) & 0xFFu) != 0 && ((r.x >> (8 * j)) & 0xFFu) == 0) << j;
+            b |= static_cast<uint32_t>(((v.y >> (8 * j)) & 0xFFu) != 0 && ((r.y >> (8 * j)) & 0xFFu) == 0) << (j + 4);
+        }
+    } else {
+        for (uint32_t j = 0; s0 + j < S; ++j) b |= static_cast<uint32_t>(valid[s0 + j] != 0 && reach[s0 + j] == 0) << j;
+    }
+    reinterpret_cast<uint8_t*>(fresh)[t] = static_cast<uint8_t>(old & ~b);
+    cb[t] = static_cast<uint8_t>(b);
+}
+
+// 32 bits of a flat bitmask from bit `pos` on (the bitmask holds one word past
+// its last bit, so both reads are in bounds for pos below the bit count).
+__device__ __forceinline__ uint32_t bits32_at(const unsigned long long* __restrict__ b, uint32_t pos) {
+    const uint32_t* const w = reinterpret_cast<const uint32_t*>(b);
+    return __builtin_amdgcn_alignbit(w[(pos >> 5) + 1], w[pos >> 5], pos & 31u);
+}
+
+// The v pass of D: per plane (θ cell z, v cell w), row y, word k of the
+// row-aligned layout, the states reached since the previous iteration ORed
+// over the planes (z, v') of w's v hull range. One thread per word.
+__global__ __launch_bounds__(256) void frontier_v_kernel(FrontierArgs F) {
+    if (loop_done(F.ctl)) return;
+    const DevCase& c = F.c;
+    const uint32_t n0 = c.shape[0], n1 = c.shape[1], n2 = c.shape[2], n3 = c.shape[3];
+    const uint32_t rw = F.row_words, nwords = n1 * rw;
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= nwords * n2 * n3) return;
+    const uint32_t plane = t / nwords, r = t - plane * nwords;
+    const uint32_t y = r / rw, k = r - y * rw;
+    const uint32_t z = plane % n2, w = plane / n2;
+    const uint2 hv = F.hull3[w];
+    const uint32_t nv = hv.y >= hv.x ? hv.y - hv.x + 1u : hv.y + n3 - hv.x + 1u;
+    const uint32_t x0 = 32u * k, len = min(32u, n0 - x0);
+    const uint32_t mask = len == 32u ? 0xFFFFFFFFu : (1u << len) - 1u;
+    const uint32_t cell = y * n0 + x0;
+    uint32_t acc = 0;
+    for (uint32_t tv = 0; tv < nv; tv += 4u) {
+        uint32_t v[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            const uint32_t pv = hv.x + tv + u >= n3 ? hv.x + tv + u - n3 : hv.x + tv + u;
+            v[u] = tv + u < nv ? bits32_at(F.fresh, (z + pv * n2) * c.PQ + cell) : 0u;
+        }
+        acc |= v[0] | v[1] | v[2] | v[3];
+    }
+    F.fresh_v[t] = acc & mask;
+}
+
+// 32 bits of LDS row `row` (rw words, bits past the row's end 0) from the
+// signed bit offset s on; bits outside the row read 0.
+__device__ __forceinline__ uint32_t row_bits_at(const uint32_t* row, int32_t rw, int32_t s) {
+    const int32_t a = s >> 5;  // floor(s / 32)
+    const uint32_t lo = (a >= 0 && a < rw) ? row[a] : 0u;
+    const uint32_t hi = (a + 1 >= 0 && a + 1 < rw) ? row[a + 1] : 0u;
+    return __builtin_amdgcn_alignbit(hi, lo, static_cast<uint32_t>(s) & 31u);
+}
+
+// One workgroup per plane: D (the v pass's rows ORed over the θ hull range),
+// then E (in LDS, row-aligned: y * rw + x / 32), then the plane's frontier
+// words and its words of the OR probe bitmask.
+__global__ __launch_bounds__(kFrontierThreads) void frontier_plane_kernel(FrontierArgs F) {
+    if (loop_done(F.ctl)) return;
+    extern __shared__ uint32_t s_rows[];
+    const DevCase& c = F.c;
+    const uint32_t plane = blockIdx.x;
+    const uint32_t n0 = c.shape[0], n1 = c.shape[1], n2 = c.shape[2];
+    const uint32_t rw = F.row_words, nwords = n1 * rw;
+    uint32_t* const A = s_rows;
+    uint32_t* const B = s_rows + nwords;
+    const int4 rect = F.plane_rect[plane];
+    const bool any_rect = rect.x <= rect.y;  // uniform
+    if (any_rect) {
+        const uint32_t z = plane % n2, w = plane / n2;
+        const uint2 hz = F.hull2[z];
+        const uint32_t nz = hz.y >= hz.x ? hz.y - hz.x + 1u : hz.y + n2 - hz.x + 1u;
+        for (uint32_t t = threadIdx.x; t < nwords; t += blockDim.x) {
+            uint32_t acc = 0;
+            for (uint32_t tz = 0; tz < nz; tz += 4u) {
+                uint32_t v[4];
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u) {
+                    const uint32_t pz = hz.x + tz + u >= n2 ? hz.x + tz + u - n2 : hz.x + tz + u;
+                    v[u] = tz + u < nz ? F.fresh_v[static_cast<size_t>(pz + w * n2) * nwords + t] : 0u;
+                }
+                acc |= v[0] | v[1] | v[2] | v[3];
+            }
+            A[t] = acc;
+        }
+        __syncthreads();
+        // x: bit x of a row = OR of the row's bits x + dx, dx in [rect.x, rect.y]
+        // (four offsets per step, their LDS reads issued together).
+        for (uint32_t t = threadIdx.x; t < nwords; t += blockDim.x) {
+            const uint32_t y = t / rw, k = t - y * rw;
+            const uint32_t len = min(32u, n0 - 32u * k);
+            const uint32_t mask = len == 32u ? 0xFFFFFFFFu : (1u << len) - 1u;
+            const uint32_t* const row = A + y * rw;
+            const int32_t s0 = static_cast<int32_t>(32u * k);
+            uint32_t acc = 0;
+            for (int32_t dx = rect.x; dx <= rect.y; dx += 4) {
+                uint32_t v[4];
+#pragma unroll
+                for (int32_t u = 0; u < 4; ++u)
+                    v[u] = dx + u <= rect.y ? row_bits_at(row, static_cast<int32_t>(rw), s0 + dx + u) : 0u;
+                acc |= v[0] | v[1] | v[2] | v[3];
+            }
+            B[t] = acc & mask;
+        }
+        __syncthreads();
+        // y: row y = OR of rows y + dy, dy in [rect.z, rect.w] (back into A).
+        for (uint32_t t = threadIdx.x; t < nwords; t += blockDim.x) {
+            const uint32_t y = t / rw, k = t - y * rw;
+            uint32_t acc = 0;
+            for (int32_t dy = rect.z; dy <= rect.w; dy += 4) {
+                uint32_t v[4];
+#pragma unroll
+                for (int32_t u = 0; u < 4; ++u) {
+                    const int32_t yy = static_cast<int32_t>(y) + dy + u;
+                    v[u] = (dy + u <= rect.w && yy >= 0 && yy < static_cast<int32_t>(n1)) ? B[yy * rw + k] : 0u;
+                }
+                acc |= v[0] | v[1] | v[2] | v[3];
+            }
+            A[t] = acc;
+        }
+        __syncthreads();
+    }
+    // The plane's chunk words, a lane per chunk: E's bits in flat cell order
+    // (row segments of at most 32 bits), kept for the states with a valid
+    // pair, with the always-tested states added; and the OR probe word.
+    const uint32_t cpp = c.chunks_per_plane;
+    for (uint32_t j = threadIdx.x; j < cpp; j += blockDim.x) {
+        const size_t k = static_cast<size_t>(plane) * cpp + j;
+        const uint64_t keep = F.keep[k], always = F.always[k];
+        const uint64_t orw = F.or_bits ? or_probe_word(c, F.cur, plane, j) : 0ull;
+        uint64_t e = 0;
+        if (any_rect) {
+            uint32_t q = j * 64u, filled = 0;
+            const uint32_t qe = min(q + 64u, c.PQ);
+            uint32_t y = q / n0, x = q - y * n0;
+            while (q < qe) {
+                const uint32_t take = min(min(32u, n0 - x), qe - q);
+                const uint32_t bits = row_bits_at(A + y * rw, static_cast<int32_t>(rw), static_cast<int32_t>(x));
+                const uint32_t m = take == 32u ? bits : bits & ((1u << take) - 1u);
+                e |= static_cast<uint64_t>(m) << filled;
+                filled += take;
+                q += take;
+                x += take;
+                if (x == n0) {
+                    x = 0;
+                    ++y;
+                }
+            }
+        }
+        F.frontier[k] = (e & keep) | always;
+        if (F.or_bits) F.or_bits[k] = orw;
+    }
+}
+
+hipError_t launch_frontier_plane(const FrontierArgs& F, hipStream_t stream) {
+    const uint64_t words = static_cast<uint64_t>(F.c.shape[1]) * F.row_words * F.c.shape[2] * F.c.shape[3];
+    hipLaunchKernelGGL(frontier_v_kernel, dim3(static_cast<uint32_t>((words + 255u) / 256u)), dim3(256), 0, stream, F);
+    hipLaunchKernelGGL(frontier_plane_kernel, dim3(F.c.shape[2] * F.c.shape[3]), dim3(kFrontierThreads),
+                       frontier_lds_bytes(F.c), stream, F);
+    return hipGetLastError();
 }
 
 // Pair satisfaction + first-input reduction with the successor boxes recomputed
@@ -1769,6 +2063,7 @@ template <uint32_t WG, uint32_t IB>
 __device__ __forceinline__ void satisfy_in_batch(const SatisfyArgs& A, uint32_t block, uint32_t iter) {
     constexpr bool kChunkFlags = true;  // chunk_flags_batch_kernel ran this iteration
     constexpr bool kSweep = false;      // the count queue in both schedules
+    constexpr bool kBits = false;       // frontier skipping on the prefix tables
 #include "satisfy_body.inc"
 }
 
@@ -1823,9 +2118,10 @@ using SatSingle = void (*)(SatisfyArgs, uint32_t);
 using SatBatch = void (*)(const BatchInstance*, const uint32_t*, uint32_t);
 
 template <uint32_t WG>
-SatSingle pick_single(uint32_t ib, bool sweep) {
-    if (sweep) return ib == 4 ? satisfy_kernel<WG, 4, true> : satisfy_kernel<WG, 8, true>;
-    return ib == 4 ? satisfy_kernel<WG, 4, false> : satisfy_kernel<WG, 8, false>;
+SatSingle pick_single(uint32_t ib, bool sweep, bool bits) {
+    if (sweep) return ib == 4 ? satisfy_kernel<WG, 4, true, false> : satisfy_kernel<WG, 8, true, false>;
+    if (bits) return ib == 4 ? satisfy_kernel<WG, 4, false, true> : satisfy_kernel<WG, 8, false, true>;
+    return ib == 4 ? satisfy_kernel<WG, 4, false, false> : satisfy_kernel<WG, 8, false, false>;
 }
 template <uint32_t WG>
 SatBatch pick_batch(uint32_t ib) {
@@ -1842,10 +2138,10 @@ uint32_t satisfy_blocks(const DevCase& c, uint32_t wg) {
 }
 
 hipError_t launch_satisfy(const SatisfyArgs& A, uint32_t iter, uint32_t wg, uint32_t ib, hipStream_t stream) {
-    const bool sw = A.sweep_all != 0;
-    const SatSingle f = wg == 128   ? pick_single<128>(ib, sw)
-                        : wg == 512 ? pick_single<512>(ib, sw)
-                                    : pick_single<256>(ib, sw);
+    const bool sw = A.sweep_all != 0, bits = A.frontier_bits != nullptr;
+    const SatSingle f = wg == 128   ? pick_single<128>(ib, sw, bits)
+                        : wg == 512 ? pick_single<512>(ib, sw, bits)
+                                    : pick_single<256>(ib, sw, bits);
     hipLaunchKernelGGL(f, dim3(satisfy_blocks(A.c, wg)), dim3(wg), 0, stream, A, iter);
     return hipGetLastError();
 }

@@ -84,6 +84,7 @@ struct AbstractionArgs {
     OverflowEntry* overflow;
     uint32_t* overflow_count;
     uint32_t overflow_capacity;
+    uint32_t* ext1_max;           // atomic max of the compact records' y extents, or null
     uint32_t* hull;               // [S] state hulls
     uint4* chunk_hull;            // [n_tasks] union of the hulls of each 64-cell chunk
 };
@@ -116,6 +117,11 @@ struct SatisfyArgs {
     // Bitmask of the valid states (flat state order, one word past the last
     // state's), written once at set-up.
     const unsigned long long* valid_bits;
+    // Plane-aligned bitmask (one word per 64-cell chunk) of the states whose
+    // hull may have seen a newly reached cell since the previous iteration,
+    // written each iteration by frontier_plane_kernel; null where frontier
+    // skipping runs on the prefix tables instead (chunk_flags, hull tests).
+    const unsigned long long* frontier_bits;
     uint32_t* rec;
     const OverflowEntry* overflow;
     const uint2* box2;
@@ -139,6 +145,30 @@ struct RecomputeArgs {
     const double* rad;
     const uint32_t* fail2d;
 };
+
+// Frontier skipping from bitmasks (single case, see frontier_plane_kernel).
+struct FrontierArgs {
+    DevCase c;
+    const unsigned long long* cur;     // valid, unreached states of this iteration (flat)
+    const unsigned long long* fresh;   // states reached since the previous iteration (flat)
+    uint32_t* fresh_v;                 // per plane, row-aligned: `fresh` ORed over the v hull range
+    const unsigned long long* keep;    // plane-aligned: states with a valid pair
+    const unsigned long long* always;  // plane-aligned: states whose hull is not representable
+    const int4* plane_rect;            // per plane: union of its states' hull offsets (x0, x1, y0, y1)
+    const uint2* hull2;
+    const uint2* hull3;
+    unsigned long long* frontier;      // out, plane-aligned
+    unsigned long long* or_bits;       // out: the OR probe bitmask, or null
+    const IterCtl* ctl;
+    uint32_t row_words;                // LDS words per y row: ceil(n_x / 32)
+};
+// LDS of frontier_plane_kernel: two row-aligned copies of one plane.
+__host__ __device__ inline uint32_t frontier_lds_bytes(const DevCase& c) {
+    return 2u * c.shape[1] * ((c.shape[0] + 31u) / 32u) * 4u;
+}
+constexpr uint32_t kFrontierLdsMax = 65536;
+constexpr uint32_t kFrontierThreads = 256;
+
 
 // One instance of a multi-instance (batched) iteration: its satisfaction
 // arguments plus what the prefix, bitmask, seed and finish stages write.
@@ -209,7 +239,7 @@ __host__ __device__ inline uint32_t probe_or_bit0(const DevCase& c) { return pro
 // (use_or: the OR bitmask is built, the chunk flag pass runs, for this case).
 __global__ void pair_rows_kernel(DevCase c, const uint2* box2, const uint2* box3, const int32_t* offs,
                                  uint32_t use_or, uint32_t* probe_base, uint32_t* row_kind, uint4* row_box,
-                                 FastRow* fast_offs, uint32_t* plane_mixed);
+                                 FastRow* fast_offs, uint32_t* plane_mixed, uint32_t* ext1_max);
 // The abstraction kernels (fast planes, then the others) over every wave task.
 hipError_t launch_abstraction_kernels(const AbstractionArgs& A, hipStream_t stream);
 __global__ void export_kernel(DevCase c, const uint32_t* rec, const uint2* box2, const uint2* box3,
@@ -227,6 +257,18 @@ hipError_t launch_satisfy(const SatisfyArgs& A, uint32_t iter, uint32_t wg, uint
 // them for iteration `iter` from the chunk hulls and the coarse tables.
 __host__ __device__ inline uint32_t chunk_flag_pitch(const DevCase& c) { return (c.chunks_per_plane + 15u) & ~15u; }
 hipError_t launch_chunk_flags(const SatisfyArgs& A, uint8_t* flags, uint32_t iter, hipStream_t stream);
+// Once per abstraction: the keep / always masks and plane hull rectangles.
+__global__ void frontier_setup_kernel(DevCase c, const uint32_t* hull, unsigned long long* keep,
+                                      unsigned long long* always, int4* plane_rect);
+// Per iteration: this iteration's unreached bitmask into `cur`, and into
+// `fresh` the states unreached before (valid, before the first iteration) and
+// reached now.
+__global__ void frontier_bits_kernel(uint32_t S, const uint8_t* valid, const uint8_t* reach, unsigned long long* cur,
+                                     unsigned long long* fresh, const unsigned long long* valid_bits, uint32_t iter,
+                                     const IterCtl* ctl);
+// Per iteration, after frontier_bits_kernel: the v pass, then one workgroup
+// per plane (frontier words and the OR probe bitmask).
+hipError_t launch_frontier_plane(const FrontierArgs& F, hipStream_t stream);
 __global__ void satisfy_recompute_kernel(RecomputeArgs R);
 
 // Multi-instance stages: `starts` holds n + 1 block offsets per stage.

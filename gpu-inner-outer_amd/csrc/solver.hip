@@ -244,9 +244,15 @@ void gsc_solver::setup_launch() {
     ubits = alloc<unsigned long long>(2ull * probe_bits_words(dc) + dc.n_tasks + 1);
     probe_or_bits = ubits + probe_bits_words(dc);
     valid_bits = probe_or_bits + dc.n_tasks + 1;
+    ubits_fresh = alloc<unsigned long long>(probe_bits_words(dc));
+    fresh_v = alloc<uint32_t>(n_planes * dc.shape[1] * ((dc.shape[0] + 31u) / 32u));
+    frontier_words = alloc<unsigned long long>(static_cast<uint64_t>(dc.n_tasks) + 1);
+    frontier_keep = alloc<unsigned long long>(dc.n_tasks);
+    frontier_always = alloc<unsigned long long>(dc.n_tasks);
+    plane_rect = alloc<int4>(n_planes);
     overflow_capacity = opts.recompute_abstraction ? 1u : 1u << 16;
     overflow = alloc<OverflowEntry>(overflow_capacity);
-    overflow_count = alloc<uint32_t>(1);
+    overflow_count = alloc<uint32_t>(2);  // escaped records, then the tallest compact box's y extent
     mask_counts = alloc<unsigned long long>(2);
     ctl = alloc<IterCtl>(1);
     records = alloc<IterRecord>(kRing);
@@ -353,6 +359,9 @@ uint64_t gsc_solver::device_bytes(const gsc_case_shape& sh, const gsc_case_desc&
     b += 3 * r(4 * ((n0 + kCoarse - 1) / kCoarse + 1) * ((n1 + kCoarse - 1) / kCoarse + 1) * (n2 + 1) * (n3 + 1));
     b += r(4 * planes * I_pad * n0 * n1) + 2 * r(4 * planes * I_pad);  // records, probe bases, row kinds
     b += r(8 * (2 * ((S + 63) / 64 + 1) + planes * ((n0 * n1 + 63) / 64) + 1));  // the three bitmasks
+    // Bitmask frontier: previous unreached bitmask, frontier, keep and always words, plane rectangles.
+    b += r(8 * ((S + 63) / 64 + 1)) + r(4 * planes * n1 * ((n0 + 31) / 32)) + r(8 * (planes * ((n0 * n1 + 63) / 64) + 1)) +
+         2 * r(8 * planes * ((n0 * n1 + 63) / 64)) + r(16 * planes);
     b += r(sizeof(OverflowEntry) * (1u << 16)) + r(4) + r(16) + r(sizeof(IterCtl)) + r(sizeof(IterRecord) * kRing);
     b += r(16 * n2 * nw * K) + r(8 * n3 * na * K) + r(8 * n2 * nw) + r(8 * n3 * na) + r(16 * n3);
     b += r(4 * (n0 + 1) * (n1 + 1)) + r(16 * planes * I * K) + r(32 * planes * I) + 2 * r(64 * 64);
@@ -406,6 +415,30 @@ void gsc_solver::upload_tables() {
     t.hull2 = cell_hulls(t.box2, dc.shape[2], dc.n_w, dc.wrap_dim == 2);
     t.hull3 = cell_hulls(t.box3, dc.shape[3], dc.n_a, dc.wrap_dim == 3);
     {
+        // The most bitmask slices a compact box's θ / v ranges need in the
+        // satisfaction's row scan: with the OR bitmask (both ranges two cells
+        // or longer) one slice per cell beyond the first on each axis, else
+        // one per cell. Ranges a compact record cannot have (rejected, or
+        // inverted off the wrap axis) are left out.
+        const auto lengths = [](const std::vector<uint2>& tab, uint32_t n, bool wraps) {
+            std::vector<uint32_t> out;
+            for (const uint2& r : tab) {
+                if (r.x == kBoxReject || (r.x > r.y && !wraps)) continue;
+                const uint32_t len = r.y >= r.x ? r.y - r.x + 1u : r.y + n - r.x + 1u;
+                if (std::find(out.begin(), out.end(), len) == out.end()) out.push_back(len);
+            }
+            return out;
+        };
+        const std::vector<uint32_t> lz = lengths(t.box2, dc.shape[2], dc.wrap_dim == 2);
+        const std::vector<uint32_t> lw = lengths(t.box3, dc.shape[3], dc.wrap_dim == 3);
+        slices_max = 0;
+        for (const uint32_t a : lz)
+            for (const uint32_t b : lw) {
+                const bool orb = probe_or_used() && a >= 2u && b >= 2u;
+                slices_max = std::max(slices_max, orb ? (a - 1u) * (b - 1u) : a * b);
+            }
+    }
+    {
         // Filter constants of the x / y fast path (set_cell_filter): every cell
         // centre lies within [lb, lb + n eta + eta], every increment within
         // |dt| max|v| (times three roundings), every radius within the table's.
@@ -446,7 +479,7 @@ void gsc_solver::upload_tables() {
     GSC_HIP(hipGetLastError());
     hipLaunchKernelGGL(pair_rows_kernel, dim3(blocks_for(n_planes, 4)), dim3(256), 0, stream, dc, box2, box3,
                        cell_offs, probe_or_used() ? 1u : 0u, probe_base, row_kind, row_box, fast_offs,
-                       plane_mixed);
+                       plane_mixed, overflow_count + 1);
     GSC_HIP(hipGetLastError());
 }
 
@@ -457,10 +490,12 @@ void gsc_solver::launch_abstraction(bool rerun) {
         abstraction_t0 = std::chrono::steady_clock::now();
         if (!events.empty()) GSC_HIP(hipEventRecord(events[0], stream));
     }
-    upload_tables();
+    // The escaped-record count and the tallest box's extent start at 0 (the
+    // per-row pass of upload_tables and the abstraction kernels add to them).
     // A fill kernel, not hipMemsetAsync, so a caller may capture this sequence.
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, overflow_count, 1u, 0u);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, overflow_count, 2u, 0u);
     GSC_HIP(hipGetLastError());
+    upload_tables();
     AbstractionArgs A{};
     A.c = dc;
     A.inc = inc;
@@ -477,11 +512,20 @@ void gsc_solver::launch_abstraction(bool rerun) {
     A.overflow = overflow;
     A.overflow_count = overflow_count;
     A.overflow_capacity = overflow_capacity;
+    A.ext1_max = overflow_count + 1;
     A.hull = hull;
     A.chunk_hull = chunk_hull;
     GSC_HIP(launch_abstraction_kernels(A, stream));
+    if (probe_or_used()) {
+        // What the bitmask frontier needs of the state hulls (used by solve()
+        // when no record escaped).
+        hipLaunchKernelGGL(frontier_setup_kernel, dim3(dc.shape[2] * dc.shape[3]), dim3(256), 0, stream, dc, hull,
+                           frontier_keep, frontier_always, plane_rect);
+        GSC_HIP(hipGetLastError());
+    }
     if (!events.empty()) GSC_HIP(hipEventRecord(events[1], stream));
-    GSC_HIP(hipMemcpyAsync(host_words + kWordOverflow, overflow_count, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    GSC_HIP(hipMemcpyAsync(host_words + kWordOverflow, overflow_count, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           stream));
 }
 
 void gsc_solver::finish_abstraction() {
@@ -500,6 +544,8 @@ void gsc_solver::finish_abstraction() {
         used = host_words[kWordOverflow];
         if (used > overflow_capacity) throw ConfigError("successor-box overflow table did not converge");
     }
+    overflow_used = used;
+    ext1_used = host_words[kWordOverflow + 1];
     abstraction_wall_ms = ms_since(abstraction_t0);
     if (events.empty()) {
         abstraction_kernel_ms = abstraction_wall_ms;  // no events without allocation: wall time
@@ -553,6 +599,22 @@ void gsc_solver::solve() {
     A.probe_base = probe_base;
     A.probe_or_bits = probe_or_used() ? probe_or_bits : nullptr;
     A.valid_bits = valid_bits;
+    const bool bits_mode = bits_frontier();
+    A.frontier_bits = bits_mode ? frontier_words : nullptr;
+    FrontierArgs FA{};
+    FA.c = dc;
+    FA.cur = ubits;
+    FA.fresh = ubits_fresh;
+    FA.fresh_v = fresh_v;
+    FA.keep = frontier_keep;
+    FA.always = frontier_always;
+    FA.plane_rect = plane_rect;
+    FA.hull2 = hull2;
+    FA.hull3 = hull3;
+    FA.frontier = frontier_words;
+    FA.or_bits = probe_or_bits;
+    FA.ctl = ctl;
+    FA.row_words = (dc.shape[0] + 31u) / 32u;
     const bool chunk_flags_used = !opts.sweep_all_pairs && dc.hull_enabled;
     A.rec = rec;
     A.overflow = overflow;
@@ -582,7 +644,14 @@ void gsc_solver::solve() {
         for (uint32_t it = first; it <= last; ++it) {
             hipEvent_t* ev = &events[(it - first) * kEventsPerIter];
             if (timed) GSC_HIP(hipEventRecord(ev[0], stream));
-            if (plane_pitch) {
+            if (bits_mode) {
+                // The unreached bitmask (keeping the previous one), then the
+                // frontier words and the OR probe bitmask.
+                hipLaunchKernelGGL(frontier_bits_kernel, dim3(blocks_for((S + 7u) / 8u, 256)), dim3(256), 0, stream, S, valid,
+                                   reach, ubits, ubits_fresh, valid_bits, it, ctl);
+                GSC_HIP(hipGetLastError());
+                GSC_HIP(launch_frontier_plane(FA, stream));
+            } else if (plane_pitch) {
                 // Plane kernel (x, y and the bitmask), θ pass, v pass with the coarse sample.
                 hipLaunchKernelGGL(prefix_plane_kernel, dim3(dc.shape[2] * dc.shape[3]), dim3(kPlaneThreads),
                                    plane_lds_bytes(plane_pitch, dc.shape[1], dc.PQ), stream, dc, valid, reach,
@@ -606,7 +675,7 @@ void gsc_solver::solve() {
             }
             // Which 64-cell chunks saw a change in their hull, for the
             // satisfaction's frontier skipping (both abstraction modes).
-            if (chunk_flags_used) GSC_HIP(launch_chunk_flags(A, chunk_flags, it, stream));
+            if (chunk_flags_used && !bits_mode) GSC_HIP(launch_chunk_flags(A, chunk_flags, it, stream));
             if (timed) GSC_HIP(hipEventRecord(ev[1], stream));
             if (opts.recompute_abstraction) {
                 hipLaunchKernelGGL(satisfy_recompute_kernel, dim3(satisfy_blocks(dc, 256)), dim3(256), 0, stream, RA);

@@ -251,6 +251,15 @@ struct gsc_solver {
     unsigned long long* ubits = nullptr;
     unsigned long long* probe_or_bits = nullptr;  // [n_tasks + 1], plane-aligned (see chunk_flags_body)
     unsigned long long* valid_bits = nullptr;     // [S / 64 + 1], the valid states (set-up)
+    // Frontier skipping from bitmasks (frontier_plane_kernel): the states
+    // reached since the previous iteration, the frontier words, the keep /
+    // always words and the plane hull rectangles.
+    unsigned long long* ubits_fresh = nullptr;    // [S / 64 + 1]: reached since the previous iteration
+    uint32_t* fresh_v = nullptr;                  // [planes * n_y * ceil(n_x / 32)] (frontier_v_kernel)
+    unsigned long long* frontier_words = nullptr;  // [n_tasks + 1], plane-aligned
+    unsigned long long* frontier_keep = nullptr;   // [n_tasks]
+    unsigned long long* frontier_always = nullptr;  // [n_tasks]
+    int4* plane_rect = nullptr;                    // [planes]
     uint32_t* probe_base = nullptr;  // [plane][I_pad], see pair_rows_kernel
     uint32_t* row_kind = nullptr;    // [plane][I_pad], see pair_rows_kernel
     uint4* row_box = nullptr;        // [plane][input], see pair_rows_kernel
@@ -367,6 +376,19 @@ struct gsc_solver {
     // frontier skipping does) and probed (by the stored-record satisfaction
     // kernels). Fixed for the solver's life: its options do not change.
     bool probe_or_used() const { return !opts.sweep_all_pairs && !opts.recompute_abstraction && dc.hull_enabled; }
+    // Frontier skipping on bitmasks instead of prefix tables: where the OR
+    // bitmask is used, every pair's box is settled by row scans of the
+    // bitmasks — no record escaped and every compact box has at most sixteen
+    // rows in at most two bitmask slices (other boxes are counted on the
+    // unreached prefix table, which this mode does not build) — and a plane
+    // fits the frontier kernel's LDS.
+    bool bits_frontier() const {
+        return probe_or_used() && overflow_used == 0 && ext1_used < 16u && slices_max <= 2u &&
+               frontier_lds_bytes(dc) <= kFrontierLdsMax;
+    }
+    uint32_t overflow_used = 0;  // escaped records of the current abstraction
+    uint32_t ext1_used = 0;      // largest y extent of its compact records
+    uint32_t slices_max = 0;     // most bitmask slices a compact box needs (upload_tables)
 
     void build_tables(std::vector<double2>& h_cs, std::vector<double>& h_vt, std::vector<uint2>& h_box2,
                       std::vector<uint2>& h_box3, std::vector<double>& h_rad, std::vector<uint32_t>& h_fail);
