@@ -20,7 +20,10 @@
 namespace gsc_amd {
 namespace {
 
-enum Stage : uint32_t { kSeed = 0, kRows, kAxis1, kAxis2, kAxis3, kCoarseStage, kBits, kChunk, kSat, kPlane, kLast, kStages };
+enum Stage : uint32_t {
+    kSeed = 0, kRows, kAxis1, kAxis2, kAxis3, kCoarseStage, kBits, kChunk, kSat, kPlane, kLast,
+    kFrontierBits, kFrontierV, kFrontierPlane, kStages
+};
 
 bool options_valid(const gsc_batch_options& o) {
     return satisfy_variant_supported(static_cast<uint32_t>(o.tile_wg), static_cast<uint32_t>(o.tile_ib)) &&
@@ -50,10 +53,14 @@ struct gsc_batch {
     // when every case's plane fits in LDS; its dynamic LDS bytes.
     bool fused_prefix = false;
     uint32_t plane_lds = 0;
+    // Bitmask frontier (frontier_*_batch_kernel) when every case qualifies
+    // (gsc_solver::bits_frontier); its dynamic LDS bytes.
+    bool bits_frontier = false;
+    uint32_t frontier_lds = 0;
     std::vector<void*> device_allocs;
 
     // One captured, instantiated graph per (tile_wg, tile_ib, iterations per
-    // launch), kept until the batch is destroyed. Its nodes read the instance
+    // launch, frontier mode), kept until the batch is destroyed. Its nodes read the instance
     // table and block ranges from d_inst / d_starts at run time, and its grid
     // sizes depend only on the tile width (the per-stage totals do not depend
     // on the case order), so neither a re-plan nor a new abstraction outdates
@@ -61,6 +68,7 @@ struct gsc_batch {
     // batch lives.
     struct GraphEntry {
         int32_t wg, ib, g;
+        bool bits;
         hipGraph_t graph;
         hipGraphExec_t exec;
     };
@@ -110,7 +118,9 @@ struct gsc_batch {
     // The cached graph of the current options; nullptr when not captured yet.
     hipGraphExec_t find_exec() const {
         for (const GraphEntry& e : graphs)
-            if (e.wg == opts.tile_wg && e.ib == opts.tile_ib && e.g == opts.iterations_per_launch) return e.exec;
+            if (e.wg == opts.tile_wg && e.ib == opts.tile_ib && e.g == opts.iterations_per_launch &&
+                e.bits == bits_frontier)
+                return e.exec;
         return nullptr;
     }
 
@@ -180,7 +190,9 @@ void gsc_batch::plan() {
     std::vector<uint32_t> starts(static_cast<size_t>(kStages) * (N + 1), 0);
     uint64_t total[kStages] = {};
     bool fused = true;
-    uint64_t lds = 0;
+    uint64_t lds = 0, flds = 0;
+    bool bits = N > 0;
+    for (uint32_t p = 0; p < N; ++p) bits = bits && cases[p]->bits_frontier();
     for (uint32_t p = 0; p < N; ++p) {
         const gsc_solver& s = *cases[order[p]];
         const DevCase& c = s.dc;
@@ -198,6 +210,11 @@ void gsc_batch::plan() {
         blocks[kPlane] = static_cast<uint64_t>(c.shape[2]) * c.shape[3];
         blocks[kLast] = (static_cast<uint64_t>(c.shape[0]) * c.shape[1] * c.shape[2] + 255) / 256;
         blocks[kSat] = satisfy_blocks(c, static_cast<uint32_t>(opts.tile_wg));
+        const uint64_t planes = static_cast<uint64_t>(c.shape[2]) * c.shape[3];
+        blocks[kFrontierBits] = ((c.S + 7ull) / 8ull + 255) / 256;
+        blocks[kFrontierV] = (planes * c.shape[1] * ((c.shape[0] + 31ull) / 32ull) + 255) / 256;
+        blocks[kFrontierPlane] = planes;
+        flds = std::max<uint64_t>(flds, frontier_lds_bytes(c));
         for (uint32_t st = 0; st < kStages; ++st) {
             starts[st * (N + 1) + p] = static_cast<uint32_t>(total[st]);
             total[st] += blocks[st];
@@ -224,6 +241,8 @@ void gsc_batch::plan() {
         // Written by chunk_flags_batch_kernel, which runs where frontier skipping does.
         b.sat.probe_or_bits = s.probe_or_used() ? s.probe_or_bits : nullptr;
         b.sat.valid_bits = s.valid_bits;
+        b.sat.frontier_bits = bits ? s.frontier_words : nullptr;
+        b.fr = s.frontier_args();
         b.sat.rec = s.rec;
         b.sat.overflow = s.overflow;
         b.sat.box2 = s.box2;
@@ -246,6 +265,8 @@ void gsc_batch::plan() {
     }
     fused_prefix = fused && N > 0;
     plane_lds = static_cast<uint32_t>(lds);
+    bits_frontier = bits;
+    frontier_lds = static_cast<uint32_t>(flds);
     for (uint32_t st = 0; st < kStages; ++st) {
         starts[st * (N + 1) + N] = static_cast<uint32_t>(total[st]);
         grid[st] = static_cast<uint32_t>(total[st]);
@@ -262,7 +283,16 @@ void gsc_batch::enqueue_iterations(uint32_t iterations) {
     const uint32_t N = n();
     const uint32_t* st = d_starts;
     for (uint32_t it = 0; it < iterations; ++it) {
-        if (fused_prefix) {
+        if (bits_frontier) {
+            // The unreached bitmask and the states reached since the last
+            // iteration, the v pass, then a workgroup per plane of every case.
+            hipLaunchKernelGGL(frontier_bits_batch_kernel, dim3(grid[kFrontierBits]), dim3(256), 0, stream, d_inst,
+                               st + kFrontierBits * (N + 1), N);
+            hipLaunchKernelGGL(frontier_v_batch_kernel, dim3(grid[kFrontierV]), dim3(256), 0, stream, d_inst,
+                               st + kFrontierV * (N + 1), N);
+            hipLaunchKernelGGL(frontier_plane_batch_kernel, dim3(grid[kFrontierPlane]), dim3(kFrontierThreads),
+                               frontier_lds, stream, d_inst, st + kFrontierPlane * (N + 1), N);
+        } else if (fused_prefix) {
             hipLaunchKernelGGL(prefix_plane_batch_kernel, dim3(grid[kPlane]), dim3(kPlaneThreads), plane_lds, stream,
                                d_inst, st + kPlane * (N + 1), N);
             hipLaunchKernelGGL(prefix_axis_batch_kernel, dim3(grid[kAxis2]), dim3(256), 0, stream, d_inst,
@@ -282,10 +312,11 @@ void gsc_batch::enqueue_iterations(uint32_t iterations) {
             hipLaunchKernelGGL(unreached_bits_batch_kernel, dim3(grid[kBits]), dim3(256), 0, stream, d_inst,
                                st + kBits * (N + 1), N);
         }
-        hipLaunchKernelGGL(chunk_flags_batch_kernel, dim3(grid[kChunk]), dim3(256), 0, stream, d_inst,
-                           st + kChunk * (N + 1), N);
+        if (!bits_frontier)
+            hipLaunchKernelGGL(chunk_flags_batch_kernel, dim3(grid[kChunk]), dim3(256), 0, stream, d_inst,
+                               st + kChunk * (N + 1), N);
         GSC_HIP(launch_satisfy_batch(d_inst, st + kSat * (N + 1), N, grid[kSat], static_cast<uint32_t>(opts.tile_wg),
-                                     static_cast<uint32_t>(opts.tile_ib), stream));
+                                     static_cast<uint32_t>(opts.tile_ib), bits_frontier, stream));
         hipLaunchKernelGGL(finish_batch_kernel, dim3(N), dim3(64), 0, stream, d_inst, N, d_status);
         GSC_HIP(hipGetLastError());
     }
@@ -317,7 +348,7 @@ void gsc_batch::solve() {
             exec = nullptr;
             throw DeviceError(e, hipGetErrorString(e));
         }
-        graphs.push_back(GraphEntry{opts.tile_wg, opts.tile_ib, opts.iterations_per_launch, graph, exec});
+        graphs.push_back(GraphEntry{opts.tile_wg, opts.tile_ib, opts.iterations_per_launch, bits_frontier, graph, exec});
     }
 
     const auto t0 = std::chrono::steady_clock::now();

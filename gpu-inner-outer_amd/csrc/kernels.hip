@@ -1613,32 +1613,31 @@ __global__ __launch_bounds__(256) void frontier_setup_kernel(DevCase c, const ui
 // Eight states per thread: their valid and reach bytes as one 8-byte load
 // each, the previous unreached byte beside them, and one byte of each bitmask
 // written (bit s of a bitmask is bit s % 8 of its byte s / 8).
-__global__ __launch_bounds__(256) void frontier_bits_kernel(uint32_t S, const uint8_t* __restrict__ valid,
-                                                            const uint8_t* __restrict__ reach,
-                                                            unsigned long long* __restrict__ cur,
-                                                            unsigned long long* __restrict__ fresh,
-                                                            const unsigned long long* __restrict__ valid_bits,
-                                                            uint32_t iter, const IterCtl* ctl) {
-    if (loop_done(ctl)) return;
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+__device__ __forceinline__ void frontier_bits_body(const FrontierArgs& F, uint32_t iter, uint32_t t) {
+    const uint32_t S = F.c.S;
     const uint32_t s0 = t * 8u;
     if (s0 >= S) return;
-    uint8_t* const cb = reinterpret_cast<uint8_t*>(cur);
-    const uint8_t old = iter == 1 ? reinterpret_cast<const uint8_t*>(valid_bits)[t] : cb[t];
+    uint8_t* const cb = reinterpret_cast<uint8_t*>(F.cur);
+    const uint8_t old = iter == 1 ? reinterpret_cast<const uint8_t*>(F.valid_bits)[t] : cb[t];
     uint32_t b = 0;
     if (s0 + 8u <= S) {
-        const uint2 v = *reinterpret_cast<const uint2*>(valid + s0);
-        const uint2 r = *reinterpret_cast<const uint2*>(reach + s0);
+        const uint2 v = *reinterpret_cast<const uint2*>(F.valid + s0);
+        const uint2 r = *reinterpret_cast<const uint2*>(F.reach + s0);
 #pragma unroll
         for (uint32_t j = 0; j < 4; ++j) {
             b |= static_cast<uint32_t>(((v.x >> (8 * j)) & 0xFFu) != 0 && ((r.x >> (8 * j)) & 0xFFu) == 0) << j;
             b |= static_cast<uint32_t>(((v.y >> (8 * j)) & 0xFFu) != 0 && ((r.y >> (8 * j)) & 0xFFu) == 0) << (j + 4);
         }
     } else {
-        for (uint32_t j = 0; s0 + j < S; ++j) b |= static_cast<uint32_t>(valid[s0 + j] != 0 && reach[s0 + j] == 0) << j;
+        for (uint32_t j = 0; s0 + j < S; ++j) b |= static_cast<uint32_t>(F.valid[s0 + j] != 0 && F.reach[s0 + j] == 0) << j;
     }
-    reinterpret_cast<uint8_t*>(fresh)[t] = static_cast<uint8_t>(old & ~b);
+    reinterpret_cast<uint8_t*>(F.fresh)[t] = static_cast<uint8_t>(old & ~b);
     cb[t] = static_cast<uint8_t>(b);
+}
+
+__global__ __launch_bounds__(256) void frontier_bits_kernel(FrontierArgs F, uint32_t iter) {
+    if (loop_done(F.ctl)) return;
+    frontier_bits_body(F, iter, blockIdx.x * 256u + threadIdx.x);
 }
 
 // 32 bits of a flat bitmask from bit `pos` on (the bitmask holds one word past
@@ -1651,12 +1650,10 @@ __device__ __forceinline__ uint32_t bits32_at(const unsigned long long* __restri
 // The v pass of D: per plane (θ cell z, v cell w), row y, word k of the
 // row-aligned layout, the states reached since the previous iteration ORed
 // over the planes (z, v') of w's v hull range. One thread per word.
-__global__ __launch_bounds__(256) void frontier_v_kernel(FrontierArgs F) {
-    if (loop_done(F.ctl)) return;
+__device__ __forceinline__ void frontier_v_body(const FrontierArgs& F, uint32_t t) {
     const DevCase& c = F.c;
     const uint32_t n0 = c.shape[0], n1 = c.shape[1], n2 = c.shape[2], n3 = c.shape[3];
     const uint32_t rw = F.row_words, nwords = n1 * rw;
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (t >= nwords * n2 * n3) return;
     const uint32_t plane = t / nwords, r = t - plane * nwords;
     const uint32_t y = r / rw, k = r - y * rw;
@@ -1679,6 +1676,11 @@ __global__ __launch_bounds__(256) void frontier_v_kernel(FrontierArgs F) {
     F.fresh_v[t] = acc & mask;
 }
 
+__global__ __launch_bounds__(256) void frontier_v_kernel(FrontierArgs F) {
+    if (loop_done(F.ctl)) return;
+    frontier_v_body(F, blockIdx.x * 256u + threadIdx.x);
+}
+
 // 32 bits of LDS row `row` (rw words, bits past the row's end 0) from the
 // signed bit offset s on; bits outside the row read 0.
 __device__ __forceinline__ uint32_t row_bits_at(const uint32_t* row, int32_t rw, int32_t s) {
@@ -1691,11 +1693,8 @@ __device__ __forceinline__ uint32_t row_bits_at(const uint32_t* row, int32_t rw,
 // One workgroup per plane: D (the v pass's rows ORed over the θ hull range),
 // then E (in LDS, row-aligned: y * rw + x / 32), then the plane's frontier
 // words and its words of the OR probe bitmask.
-__global__ __launch_bounds__(kFrontierThreads) void frontier_plane_kernel(FrontierArgs F) {
-    if (loop_done(F.ctl)) return;
-    extern __shared__ uint32_t s_rows[];
+__device__ __forceinline__ void frontier_plane_body(const FrontierArgs& F, uint32_t plane, uint32_t* s_rows) {
     const DevCase& c = F.c;
-    const uint32_t plane = blockIdx.x;
     const uint32_t n0 = c.shape[0], n1 = c.shape[1], n2 = c.shape[2];
     const uint32_t rw = F.row_words, nwords = n1 * rw;
     uint32_t* const A = s_rows;
@@ -1786,6 +1785,12 @@ __global__ __launch_bounds__(kFrontierThreads) void frontier_plane_kernel(Fronti
         F.frontier[k] = (e & keep) | always;
         if (F.or_bits) F.or_bits[k] = orw;
     }
+}
+
+__global__ __launch_bounds__(kFrontierThreads) void frontier_plane_kernel(FrontierArgs F) {
+    if (loop_done(F.ctl)) return;
+    extern __shared__ uint32_t s_rows[];
+    frontier_plane_body(F, blockIdx.x, s_rows);
 }
 
 hipError_t launch_frontier_plane(const FrontierArgs& F, hipStream_t stream) {
@@ -2057,24 +2062,55 @@ __global__ __launch_bounds__(256) void chunk_flags_batch_kernel(const BatchInsta
                      (blockIdx.x - starts[k]) * 256u + threadIdx.x);
 }
 
+// The bitmask frontier of every instance (frontier_bits_kernel,
+// frontier_v_kernel, frontier_plane_kernel), each case's arguments in
+// BatchInstance::fr; the plane stage's LDS is sized for the largest case.
+__global__ __launch_bounds__(256) void frontier_bits_batch_kernel(const BatchInstance* __restrict__ B,
+                                                                  const uint32_t* __restrict__ starts, uint32_t n) {
+    const uint32_t k = find_instance(starts, n, blockIdx.x);
+    const BatchInstance& I = B[k];
+    const uint64_t st = loop_state(I.sat.ctl);
+    if (state_done(st)) return;
+    frontier_bits_body(I.fr, state_next_iteration(st), (blockIdx.x - starts[k]) * 256u + threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void frontier_v_batch_kernel(const BatchInstance* __restrict__ B,
+                                                               const uint32_t* __restrict__ starts, uint32_t n) {
+    const uint32_t k = find_instance(starts, n, blockIdx.x);
+    const BatchInstance& I = B[k];
+    if (loop_done(I.sat.ctl)) return;
+    frontier_v_body(I.fr, (blockIdx.x - starts[k]) * 256u + threadIdx.x);
+}
+
+__global__ __launch_bounds__(kFrontierThreads) void frontier_plane_batch_kernel(const BatchInstance* __restrict__ B,
+                                                                                const uint32_t* __restrict__ starts,
+                                                                                uint32_t n) {
+    const uint32_t k = find_instance(starts, n, blockIdx.x);
+    const BatchInstance& I = B[k];
+    if (loop_done(I.sat.ctl)) return;
+    extern __shared__ uint32_t s_rows[];
+    frontier_plane_body(I.fr, blockIdx.x - starts[k], s_rows);
+}
+
 // The batched kernel reads its case's arguments from global memory; there the
 // function boundary gives the better register allocation (74 vs 97 VGPRs).
-template <uint32_t WG, uint32_t IB>
+// kBits: the batch runs the bitmask frontier (every case qualifies), else the
+// prefix-table stages and chunk flags.
+template <uint32_t WG, uint32_t IB, bool kBits>
 __device__ __forceinline__ void satisfy_in_batch(const SatisfyArgs& A, uint32_t block, uint32_t iter) {
-    constexpr bool kChunkFlags = true;  // chunk_flags_batch_kernel ran this iteration
+    constexpr bool kChunkFlags = true;  // this iteration's chunk flags or frontier words are written
     constexpr bool kSweep = false;      // the count queue in both schedules
-    constexpr bool kBits = false;       // frontier skipping on the prefix tables
 #include "satisfy_body.inc"
 }
 
-template <uint32_t WG, uint32_t IB>
+template <uint32_t WG, uint32_t IB, bool kBits>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) void satisfy_batch_kernel(const BatchInstance* __restrict__ B,
                                                            const uint32_t* __restrict__ starts, uint32_t n) {
     const uint32_t inst = find_instance(starts, n, blockIdx.x);
     const SatisfyArgs& A = B[inst].sat;
     const uint64_t st = loop_state(A.ctl);
     if (state_done(st)) return;
-    satisfy_in_batch<WG, IB>(A, blockIdx.x - starts[inst], state_next_iteration(st));
+    satisfy_in_batch<WG, IB, kBits>(A, blockIdx.x - starts[inst], state_next_iteration(st));
 }
 
 __global__ void finish_batch_kernel(const BatchInstance* __restrict__ B, uint32_t n, BatchStatus* status) {
@@ -2124,8 +2160,9 @@ SatSingle pick_single(uint32_t ib, bool sweep, bool bits) {
     return ib == 4 ? satisfy_kernel<WG, 4, false, false> : satisfy_kernel<WG, 8, false, false>;
 }
 template <uint32_t WG>
-SatBatch pick_batch(uint32_t ib) {
-    return ib == 4 ? satisfy_batch_kernel<WG, 4> : satisfy_batch_kernel<WG, 8>;
+SatBatch pick_batch(uint32_t ib, bool bits) {
+    if (bits) return ib == 4 ? satisfy_batch_kernel<WG, 4, true> : satisfy_batch_kernel<WG, 8, true>;
+    return ib == 4 ? satisfy_batch_kernel<WG, 4, false> : satisfy_batch_kernel<WG, 8, false>;
 }
 }  // namespace
 
@@ -2159,8 +2196,10 @@ hipError_t launch_chunk_flags(const SatisfyArgs& A, uint8_t* flags, uint32_t ite
 }
 
 hipError_t launch_satisfy_batch(const BatchInstance* B, const uint32_t* starts, uint32_t n, uint32_t blocks,
-                                uint32_t wg, uint32_t ib, hipStream_t stream) {
-    const SatBatch f = wg == 128 ? pick_batch<128>(ib) : wg == 512 ? pick_batch<512>(ib) : pick_batch<256>(ib);
+                                uint32_t wg, uint32_t ib, bool bits, hipStream_t stream) {
+    const SatBatch f = wg == 128   ? pick_batch<128>(ib, bits)
+                       : wg == 512 ? pick_batch<512>(ib, bits)
+                                   : pick_batch<256>(ib, bits);
     hipLaunchKernelGGL(f, dim3(blocks), dim3(wg), 0, stream, B, starts, n);
     return hipGetLastError();
 }

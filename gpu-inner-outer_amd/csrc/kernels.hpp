@@ -149,8 +149,11 @@ struct RecomputeArgs {
 // Frontier skipping from bitmasks (single case, see frontier_plane_kernel).
 struct FrontierArgs {
     DevCase c;
-    const unsigned long long* cur;     // valid, unreached states of this iteration (flat)
-    const unsigned long long* fresh;   // states reached since the previous iteration (flat)
+    const uint8_t* valid;
+    const uint8_t* reach;
+    const unsigned long long* valid_bits;
+    unsigned long long* cur;           // valid, unreached states of this iteration (flat)
+    unsigned long long* fresh;         // states reached since the previous iteration (flat)
     uint32_t* fresh_v;                 // per plane, row-aligned: `fresh` ORed over the v hull range
     const unsigned long long* keep;    // plane-aligned: states with a valid pair
     const unsigned long long* always;  // plane-aligned: states whose hull is not representable
@@ -174,6 +177,7 @@ constexpr uint32_t kFrontierThreads = 256;
 // arguments plus what the prefix, bitmask, seed and finish stages write.
 struct BatchInstance {
     SatisfyArgs sat;
+    FrontierArgs fr;  // the bitmask frontier's arguments (batches that run it)
     uint32_t* prefix_unreached[2];
     uint32_t* coarse_unreached[2];
     unsigned long long* unreached_bits;
@@ -262,10 +266,8 @@ __global__ void frontier_setup_kernel(DevCase c, const uint32_t* hull, unsigned 
                                       unsigned long long* always, int4* plane_rect);
 // Per iteration: this iteration's unreached bitmask into `cur`, and into
 // `fresh` the states unreached before (valid, before the first iteration) and
-// reached now.
-__global__ void frontier_bits_kernel(uint32_t S, const uint8_t* valid, const uint8_t* reach, unsigned long long* cur,
-                                     unsigned long long* fresh, const unsigned long long* valid_bits, uint32_t iter,
-                                     const IterCtl* ctl);
+// reached now; eight states per thread.
+__global__ void frontier_bits_kernel(FrontierArgs F, uint32_t iter);
 // Per iteration, after frontier_bits_kernel: the v pass, then one workgroup
 // per plane (frontier words and the OR probe bitmask).
 hipError_t launch_frontier_plane(const FrontierArgs& F, hipStream_t stream);
@@ -275,12 +277,15 @@ __global__ void satisfy_recompute_kernel(RecomputeArgs R);
 __global__ void prefix_rows_batch_kernel(const BatchInstance* B, const uint32_t* starts, uint32_t n);
 __global__ void prefix_axis_batch_kernel(const BatchInstance* B, const uint32_t* starts, uint32_t n, int axis);
 __global__ void unreached_bits_batch_kernel(const BatchInstance* B, const uint32_t* starts, uint32_t n);
+__global__ void frontier_bits_batch_kernel(const BatchInstance* B, const uint32_t* starts, uint32_t n);
+__global__ void frontier_v_batch_kernel(const BatchInstance* B, const uint32_t* starts, uint32_t n);
+__global__ void frontier_plane_batch_kernel(const BatchInstance* B, const uint32_t* starts, uint32_t n);
 __global__ void finish_batch_kernel(const BatchInstance* B, uint32_t n, BatchStatus* status);
 __global__ void seed_batch_kernel(const BatchInstance* B, const uint32_t* starts, uint32_t n);
 __global__ void chunk_flags_batch_kernel(const BatchInstance* B, const uint32_t* starts, uint32_t n);
 __global__ void prefix_plane_batch_kernel(const BatchInstance* B, const uint32_t* starts, uint32_t n);
 __global__ void prefix_last_axis_batch_kernel(const BatchInstance* B, const uint32_t* starts, uint32_t n);
 hipError_t launch_satisfy_batch(const BatchInstance* B, const uint32_t* starts, uint32_t n, uint32_t blocks,
-                                uint32_t wg, uint32_t ib, hipStream_t stream);
+                                uint32_t wg, uint32_t ib, bool bits, hipStream_t stream);
 
 }  // namespace gsc_amd

@@ -297,7 +297,7 @@ void gsc_server::enqueue_step() {
         hipLaunchKernelGGL(chunk_flags_batch_kernel, dim3(grid[kChunk]), dim3(256), 0, iter_stream, d_inst,
                            st + kChunk * (N + 1), N);
         GSC_HIP(launch_satisfy_batch(d_inst, st + kSat * (N + 1), N, grid[kSat], static_cast<uint32_t>(opts.tile_wg),
-                                     static_cast<uint32_t>(opts.tile_ib), iter_stream));
+                                     static_cast<uint32_t>(opts.tile_ib), false, iter_stream));
         hipLaunchKernelGGL(finish_batch_kernel, dim3(N), dim3(64), 0, iter_stream, d_inst, N, d_status);
         GSC_HIP(hipGetLastError());
     }

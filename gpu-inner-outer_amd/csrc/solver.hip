@@ -557,6 +557,27 @@ void gsc_solver::finish_abstraction() {
     abstraction_ready = true;
 }
 
+FrontierArgs gsc_solver::frontier_args() const {
+    FrontierArgs FA{};
+    FA.c = dc;
+    FA.valid = valid;
+    FA.reach = reach;
+    FA.valid_bits = valid_bits;
+    FA.cur = ubits;
+    FA.fresh = ubits_fresh;
+    FA.fresh_v = fresh_v;
+    FA.keep = frontier_keep;
+    FA.always = frontier_always;
+    FA.plane_rect = plane_rect;
+    FA.hull2 = hull2;
+    FA.hull3 = hull3;
+    FA.frontier = frontier_words;
+    FA.or_bits = probe_or_bits;
+    FA.ctl = ctl;
+    FA.row_words = (dc.shape[0] + 31u) / 32u;
+    return FA;
+}
+
 void gsc_solver::solve() {
     if (!abstraction_ready) build_abstraction();
     if (!host_ctl) {
@@ -601,20 +622,7 @@ void gsc_solver::solve() {
     A.valid_bits = valid_bits;
     const bool bits_mode = bits_frontier();
     A.frontier_bits = bits_mode ? frontier_words : nullptr;
-    FrontierArgs FA{};
-    FA.c = dc;
-    FA.cur = ubits;
-    FA.fresh = ubits_fresh;
-    FA.fresh_v = fresh_v;
-    FA.keep = frontier_keep;
-    FA.always = frontier_always;
-    FA.plane_rect = plane_rect;
-    FA.hull2 = hull2;
-    FA.hull3 = hull3;
-    FA.frontier = frontier_words;
-    FA.or_bits = probe_or_bits;
-    FA.ctl = ctl;
-    FA.row_words = (dc.shape[0] + 31u) / 32u;
+    const FrontierArgs FA = frontier_args();
     const bool chunk_flags_used = !opts.sweep_all_pairs && dc.hull_enabled;
     A.rec = rec;
     A.overflow = overflow;
@@ -647,8 +655,8 @@ void gsc_solver::solve() {
             if (bits_mode) {
                 // The unreached bitmask (keeping the previous one), then the
                 // frontier words and the OR probe bitmask.
-                hipLaunchKernelGGL(frontier_bits_kernel, dim3(blocks_for((S + 7u) / 8u, 256)), dim3(256), 0, stream, S, valid,
-                                   reach, ubits, ubits_fresh, valid_bits, it, ctl);
+                hipLaunchKernelGGL(frontier_bits_kernel, dim3(blocks_for((S + 7u) / 8u, 256)), dim3(256), 0, stream, FA,
+                                   it);
                 GSC_HIP(hipGetLastError());
                 GSC_HIP(launch_frontier_plane(FA, stream));
             } else if (plane_pitch) {

@@ -382,6 +382,8 @@ struct gsc_solver {
     // rows in at most two bitmask slices (other boxes are counted on the
     // unreached prefix table, which this mode does not build) — and a plane
     // fits the frontier kernel's LDS.
+    // The bitmask frontier's kernel arguments for this case.
+    FrontierArgs frontier_args() const;
     bool bits_frontier() const {
         return probe_or_used() && overflow_used == 0 && ext1_used < 16u && slices_max <= 2u &&
                frontier_lds_bytes(dc) <= kFrontierLdsMax;
