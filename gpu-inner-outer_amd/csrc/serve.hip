@@ -20,7 +20,15 @@
 namespace gsc_amd {
 namespace {
 
-enum Stage : uint32_t { kSeed = 0, kRows, kAxis1, kAxis2, kAxis3, kCoarseStage, kBits, kChunk, kSat, kPlane, kLast, kStages };
+// The prefix-table stages (kRows .. kLast, kChunk, kSat) cover the cases that
+// run frontier skipping on prefix tables, the kFrontier* stages and kSatBits
+// the cases on the bitmask frontier (gsc_solver::bits_frontier): each case
+// keeps its mode for its whole solve, and a stage's block ranges give the
+// other mode's cases no blocks.
+enum Stage : uint32_t {
+    kSeed = 0, kRows, kAxis1, kAxis2, kAxis3, kCoarseStage, kBits, kChunk, kSat, kPlane, kLast,
+    kFrontierBits, kFrontierV, kFrontierPlane, kSatBits, kStages
+};
 
 using Clock = std::chrono::steady_clock;
 
@@ -81,6 +89,10 @@ struct gsc_server {
     // when every case's plane fits in LDS; its dynamic LDS bytes.
     bool fused_prefix = false;
     uint32_t plane_lds = 0;
+    // Cases of the current step in each frontier mode, and the bitmask
+    // frontier's dynamic LDS bytes.
+    uint32_t n_prefix_mode = 0, n_bits_mode = 0;
+    uint32_t frontier_lds = 0;
 
     std::mutex m;
     std::condition_variable cv_work, cv_done;
@@ -191,10 +203,13 @@ void gsc_server::plan() {
     std::vector<uint32_t> starts(static_cast<size_t>(kStages) * (N + 1), 0);
     uint64_t total[kStages] = {};
     bool fused = true;
-    uint64_t lds = 0;
+    uint64_t lds = 0, flds = 0;
+    uint32_t n_bits = 0;
     for (uint32_t p = 0; p < N; ++p) {
         const gsc_solver& s = *active[p]->solver;
         const DevCase& c = s.dc;
+        const bool bits = s.bits_frontier();
+        n_bits += bits ? 1u : 0u;
         uint64_t blocks[kStages];
         blocks[kSeed] = (c.S + 255) / 256;
         blocks[kRows] = (static_cast<uint64_t>(c.shape[1]) * c.shape[2] * c.shape[3] + 3) / 4;
@@ -209,6 +224,17 @@ void gsc_server::plan() {
         blocks[kPlane] = static_cast<uint64_t>(c.shape[2]) * c.shape[3];
         blocks[kLast] = (static_cast<uint64_t>(c.shape[0]) * c.shape[1] * c.shape[2] + 255) / 256;
         blocks[kSat] = satisfy_blocks(c, static_cast<uint32_t>(opts.tile_wg));
+        const uint64_t planes = static_cast<uint64_t>(c.shape[2]) * c.shape[3];
+        blocks[kFrontierBits] = ((c.S + 7ull) / 8ull + 255) / 256;
+        blocks[kFrontierV] = (planes * c.shape[1] * ((c.shape[0] + 31ull) / 32ull) + 255) / 256;
+        blocks[kFrontierPlane] = planes;
+        blocks[kSatBits] = blocks[kSat];
+        // Only the stages of the case's own mode get blocks.
+        for (uint32_t st = kRows; st < kStages; ++st) {
+            const bool bits_stage = st == kFrontierBits || st == kFrontierV || st == kFrontierPlane || st == kSatBits;
+            if (bits_stage != bits) blocks[st] = 0;
+        }
+        if (bits) flds = std::max<uint64_t>(flds, frontier_lds_bytes(c));
         for (uint32_t st = 0; st < kStages; ++st) {
             starts[st * (N + 1) + p] = static_cast<uint32_t>(total[st]);
             total[st] += blocks[st];
@@ -225,8 +251,10 @@ void gsc_server::plan() {
         b.sat.chunk_hull = s.chunk_hull;
         b.sat.chunk_flags = s.chunk_flags;
         b.plane_pitch = s.plane_pitch;
-        fused = fused && s.plane_pitch != 0;
-        lds = std::max<uint64_t>(lds, plane_lds_bytes(s.plane_pitch, c.shape[1], c.PQ));
+        if (!bits) {
+            fused = fused && s.plane_pitch != 0;
+            lds = std::max<uint64_t>(lds, plane_lds_bytes(s.plane_pitch, c.shape[1], c.PQ));
+        }
         b.sat.hull2 = s.hull2;
         b.sat.hull3 = s.hull3;
         b.sat.prefix_valid = s.prefix_valid;
@@ -235,6 +263,8 @@ void gsc_server::plan() {
         // Written by chunk_flags_batch_kernel, which runs where frontier skipping does.
         b.sat.probe_or_bits = s.probe_or_used() ? s.probe_or_bits : nullptr;
         b.sat.valid_bits = s.valid_bits;
+        b.sat.frontier_bits = bits ? s.frontier_words : nullptr;
+        b.fr = s.frontier_args();
         b.sat.rec = s.rec;
         b.sat.overflow = s.overflow;
         b.sat.box2 = s.box2;
@@ -255,8 +285,11 @@ void gsc_server::plan() {
         b.candidate_count = s.candidate_count;
         b.max_iterations = s.cfg.max_iterations;
     }
-    fused_prefix = fused && N > 0;
+    n_bits_mode = n_bits;
+    n_prefix_mode = N - n_bits;
+    fused_prefix = fused && n_prefix_mode > 0;
     plane_lds = static_cast<uint32_t>(lds);
+    frontier_lds = static_cast<uint32_t>(flds);
     for (uint32_t st = 0; st < kStages; ++st) {
         starts[st * (N + 1) + N] = static_cast<uint32_t>(total[st]);
         grid[st] = static_cast<uint32_t>(total[st]);
@@ -274,30 +307,43 @@ void gsc_server::enqueue_step() {
     const uint32_t N = static_cast<uint32_t>(active.size());
     const uint32_t* st = d_starts;
     for (int32_t it = 0; it < opts.iterations_per_step; ++it) {
-        if (fused_prefix) {
-            hipLaunchKernelGGL(prefix_plane_batch_kernel, dim3(grid[kPlane]), dim3(kPlaneThreads), plane_lds, iter_stream,
-                               d_inst, st + kPlane * (N + 1), N);
-            hipLaunchKernelGGL(prefix_axis_batch_kernel, dim3(grid[kAxis2]), dim3(256), 0, iter_stream, d_inst,
-                               st + kAxis2 * (N + 1), N, 2);
-            hipLaunchKernelGGL(prefix_last_axis_batch_kernel, dim3(grid[kLast]), dim3(256), 0, iter_stream, d_inst,
-                               st + kLast * (N + 1), N);
-        } else {
-            hipLaunchKernelGGL(prefix_rows_batch_kernel, dim3(grid[kRows]), dim3(256), 0, iter_stream, d_inst,
-                               st + kRows * (N + 1), N);
-            for (int axis = 1; axis <= 3; ++axis) {
-                const uint32_t s = kAxis1 + axis - 1;
-                hipLaunchKernelGGL(prefix_axis_batch_kernel, dim3(grid[s]), dim3(256), 0, iter_stream, d_inst,
-                                   st + s * (N + 1), N, axis);
+        if (n_prefix_mode) {
+            if (fused_prefix) {
+                hipLaunchKernelGGL(prefix_plane_batch_kernel, dim3(grid[kPlane]), dim3(kPlaneThreads), plane_lds, iter_stream,
+                                   d_inst, st + kPlane * (N + 1), N);
+                hipLaunchKernelGGL(prefix_axis_batch_kernel, dim3(grid[kAxis2]), dim3(256), 0, iter_stream, d_inst,
+                                   st + kAxis2 * (N + 1), N, 2);
+                hipLaunchKernelGGL(prefix_last_axis_batch_kernel, dim3(grid[kLast]), dim3(256), 0, iter_stream, d_inst,
+                                   st + kLast * (N + 1), N);
+            } else {
+                hipLaunchKernelGGL(prefix_rows_batch_kernel, dim3(grid[kRows]), dim3(256), 0, iter_stream, d_inst,
+                                   st + kRows * (N + 1), N);
+                for (int axis = 1; axis <= 3; ++axis) {
+                    const uint32_t s = kAxis1 + axis - 1;
+                    hipLaunchKernelGGL(prefix_axis_batch_kernel, dim3(grid[s]), dim3(256), 0, iter_stream, d_inst,
+                                       st + s * (N + 1), N, axis);
+                }
+                hipLaunchKernelGGL(coarse_batch_kernel, dim3(grid[kCoarseStage]), dim3(256), 0, iter_stream, d_inst,
+                                   st + kCoarseStage * (N + 1), N);
+                hipLaunchKernelGGL(unreached_bits_batch_kernel, dim3(grid[kBits]), dim3(256), 0, iter_stream, d_inst,
+                                   st + kBits * (N + 1), N);
             }
-            hipLaunchKernelGGL(coarse_batch_kernel, dim3(grid[kCoarseStage]), dim3(256), 0, iter_stream, d_inst,
-                               st + kCoarseStage * (N + 1), N);
-            hipLaunchKernelGGL(unreached_bits_batch_kernel, dim3(grid[kBits]), dim3(256), 0, iter_stream, d_inst,
-                               st + kBits * (N + 1), N);
+            hipLaunchKernelGGL(chunk_flags_batch_kernel, dim3(grid[kChunk]), dim3(256), 0, iter_stream, d_inst,
+                               st + kChunk * (N + 1), N);
+            GSC_HIP(launch_satisfy_batch(d_inst, st + kSat * (N + 1), N, grid[kSat], static_cast<uint32_t>(opts.tile_wg),
+                                         static_cast<uint32_t>(opts.tile_ib), false, iter_stream));
         }
-        hipLaunchKernelGGL(chunk_flags_batch_kernel, dim3(grid[kChunk]), dim3(256), 0, iter_stream, d_inst,
-                           st + kChunk * (N + 1), N);
-        GSC_HIP(launch_satisfy_batch(d_inst, st + kSat * (N + 1), N, grid[kSat], static_cast<uint32_t>(opts.tile_wg),
-                                     static_cast<uint32_t>(opts.tile_ib), false, iter_stream));
+        if (n_bits_mode) {
+            hipLaunchKernelGGL(frontier_bits_batch_kernel, dim3(grid[kFrontierBits]), dim3(256), 0, iter_stream, d_inst,
+                               st + kFrontierBits * (N + 1), N);
+            hipLaunchKernelGGL(frontier_v_batch_kernel, dim3(grid[kFrontierV]), dim3(256), 0, iter_stream, d_inst,
+                               st + kFrontierV * (N + 1), N);
+            hipLaunchKernelGGL(frontier_plane_batch_kernel, dim3(grid[kFrontierPlane]), dim3(kFrontierThreads),
+                               frontier_lds, iter_stream, d_inst, st + kFrontierPlane * (N + 1), N);
+            GSC_HIP(launch_satisfy_batch(d_inst, st + kSatBits * (N + 1), N, grid[kSatBits],
+                                         static_cast<uint32_t>(opts.tile_wg), static_cast<uint32_t>(opts.tile_ib), true,
+                                         iter_stream));
+        }
         hipLaunchKernelGGL(finish_batch_kernel, dim3(N), dim3(64), 0, iter_stream, d_inst, N, d_status);
         GSC_HIP(hipGetLastError());
     }
