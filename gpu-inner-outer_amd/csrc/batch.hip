@@ -415,6 +415,8 @@ void gsc_batch::solve() {
         cs.boxes_counted = bc;
         cs.test_slots = ts;
         cs.pair_bytes_read = pt * sizeof(uint32_t);
+        // The batch runs the bitmask frontier only when every case can.
+        cs.frontier = s.frontier_mode() == GSC_FRONTIER_BITS && !bits_frontier ? GSC_FRONTIER_PREFIX : s.frontier_mode();
         s.summary = cs;
         s.iter_stats = stats[k];
         s.timings = gsc_kernel_timings{};

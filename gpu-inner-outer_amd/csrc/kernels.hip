@@ -1181,7 +1181,8 @@ __global__ __launch_bounds__(256) void export_kernel(DevCase c, const uint32_t* 
 // ---------------------------------------------------------------------------
 // Pair satisfaction by prefix counts, for the boxes the row scans do not
 // cover: an escaped pair, or a compact box of more than sixteen rows or more
-// than two bitmask slices (only where the unreached prefix table is built).
+// than two bitmask slices along θ or v (only where the unreached prefix
+// table is built).
 //
 // sat = !dead && count(unreached valid cells in the box) == 0, which equals the
 // reference's `valid && cntV > 0 && cntR == cntV` because reachable ⊆ valid
@@ -1254,9 +1255,9 @@ __device__ __forceinline__ bool rows_any_16(const uint32_t* __restrict__ bm, uin
 // pair must be shown free of unreached valid cells everywhere in its box, so
 // the test scans the whole box where that is a few rows: the record, the
 // probe base and the θ / v ranges are loaded together, then every x-row of
-// the box (up to two θ / v slices of the probe bitmask, up to sixteen rows
-// each, a round of eight row loads at a time) from the cache-resident
-// bitmask, and finally one row of
+// the box (up to two slices of the probe bitmask along θ and two along v, up
+// to sixteen rows each, a round of eight row loads at a time) from the
+// cache-resident bitmask, and finally one row of
 // the valid bitmask, which shows a valid cell in the box in almost every case.
 // Other boxes take two more rows of the probe bitmask (the first and last y
 // rows, where the unreached cells left in a box tend to sit) and the
@@ -1290,15 +1291,17 @@ __device__ __forceinline__ bool pair_satisfied(const SatisfyArgs& A, const uint3
         const uint32_t nz = range_cells(b2, n2), nw = range_cells(b3, n3);
         const bool orb = pbi >= probe_or_bit0(c);
         const uint32_t sz = orb ? nz - 1u : nz, sw = orb ? nw - 1u : nw;
-        if (kBits || (e1 < 16u && sz * sw <= 2u)) {
+        if (kBits || (e1 < 16u && sz <= 2u && sw <= 2u)) {
             const uint32_t pitch = orb ? probe_or_pitch(c) : c.PQ;
-            // The second slice: one θ cell up (sz = 2) or one v cell up (sw = 2).
+            // The other slices start one θ cell up (sz = 2), one v cell up
+            // (sw = 2), or both.
             const uint32_t z1 = b2.x + 1u == n2 ? 0u : b2.x + 1u, w1 = b3.x + 1u == n3 ? 0u : b3.x + 1u;
             const uint32_t p0 = b2.x + b3.x * n2;
-            const uint32_t p1 = sz == 2u ? z1 + b3.x * n2 : sw == 2u ? b2.x + w1 * n2 : p0;
             const uint32_t a = first + pbi;
             if (rows_any_16(bm, a, st1, e1, sh)) return false;
-            if (p1 != p0 && rows_any_16(bm, a + (p1 - p0) * pitch, st1, e1, sh)) return false;
+            if (sz == 2u && rows_any_16(bm, a + (z1 + b3.x * n2 - p0) * pitch, st1, e1, sh)) return false;
+            if (sw == 2u && rows_any_16(bm, a + (b2.x + w1 * n2 - p0) * pitch, st1, e1, sh)) return false;
+            if (sz == 2u && sw == 2u && rows_any_16(bm, a + (z1 + w1 * n2 - p0) * pitch, st1, e1, sh)) return false;
             // No unreached valid cell in the box: satisfied unless it holds no
             // valid cell at all. One row of the valid bitmask (the middle row at
             // the lower θ and v) almost always shows one; otherwise the count

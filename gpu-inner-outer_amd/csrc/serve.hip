@@ -447,6 +447,7 @@ void gsc_server::retire(Request* r, const BatchStatus& st) {
     cs.boxes_counted = bc;
     cs.test_slots = ts;
     cs.pair_bytes_read = pt * sizeof(uint32_t);
+    cs.frontier = s.frontier_mode();
     s.summary = cs;
     s.iter_stats = r->stats;
     s.timings = gsc_kernel_timings{};

@@ -415,10 +415,9 @@ void gsc_solver::upload_tables() {
     t.hull2 = cell_hulls(t.box2, dc.shape[2], dc.n_w, dc.wrap_dim == 2);
     t.hull3 = cell_hulls(t.box3, dc.shape[3], dc.n_a, dc.wrap_dim == 3);
     {
-        // The most bitmask slices a compact box's θ / v ranges need in the
+        // The most bitmask slices a compact box's θ or v range needs in the
         // satisfaction's row scan: with the OR bitmask (both ranges two cells
-        // or longer) one slice per cell beyond the first on each axis, else
-        // one per cell. Ranges a compact record cannot have (rejected, or
+        // or longer) one slice per cell beyond the first, else one per cell. Ranges a compact record cannot have (rejected, or
         // inverted off the wrap axis) are left out.
         const auto lengths = [](const std::vector<uint2>& tab, uint32_t n, bool wraps) {
             std::vector<uint32_t> out;
@@ -435,7 +434,7 @@ void gsc_solver::upload_tables() {
         for (const uint32_t a : lz)
             for (const uint32_t b : lw) {
                 const bool orb = probe_or_used() && a >= 2u && b >= 2u;
-                slices_max = std::max(slices_max, orb ? (a - 1u) * (b - 1u) : a * b);
+                slices_max = std::max(slices_max, std::max(orb ? a - 1u : a, orb ? b - 1u : b));
             }
     }
     {
@@ -750,6 +749,7 @@ void gsc_solver::solve() {
     summary.satisfaction_ms = sat_ms;
     summary.reduction_ms = red_ms;
     summary.pair_bytes_read = summary.pairs_tested * sizeof(uint32_t);
+    summary.frontier = frontier_mode();
     if (max_it == 0) {
         summary.iterations = 0;
         summary.converged = 0;

@@ -379,9 +379,14 @@ struct gsc_solver {
     // Frontier skipping on bitmasks instead of prefix tables: where the OR
     // bitmask is used, every pair's box is settled by row scans of the
     // bitmasks — no record escaped and every compact box has at most sixteen
-    // rows in at most two bitmask slices (other boxes are counted on the
+    // rows in at most two bitmask slices along θ and along v (other boxes are counted on the
     // unreached prefix table, which this mode does not build) — and a plane
     // fits the frontier kernel's LDS.
+    // The frontier skipping a solve of this case runs (GSC_FRONTIER_*).
+    uint32_t frontier_mode() const {
+        if (opts.sweep_all_pairs || !dc.hull_enabled) return GSC_FRONTIER_NONE;
+        return bits_frontier() ? GSC_FRONTIER_BITS : GSC_FRONTIER_PREFIX;
+    }
     // The bitmask frontier's kernel arguments for this case.
     FrontierArgs frontier_args() const;
     bool bits_frontier() const {
@@ -390,7 +395,7 @@ struct gsc_solver {
     }
     uint32_t overflow_used = 0;  // escaped records of the current abstraction
     uint32_t ext1_used = 0;      // largest y extent of its compact records
-    uint32_t slices_max = 0;     // most bitmask slices a compact box needs (upload_tables)
+    uint32_t slices_max = 0;     // most bitmask slices along θ or v a compact box needs (upload_tables)
 
     void build_tables(std::vector<double2>& h_cs, std::vector<double>& h_vt, std::vector<uint2>& h_box2,
                       std::vector<uint2>& h_box3, std::vector<double>& h_rad, std::vector<uint32_t>& h_fail);

@@ -220,6 +220,8 @@ class SolveSummary(ctypes.Structure):
         ("boxes_counted", c_uint64),
         ("test_slots", c_uint64),
         ("message", c_char * MESSAGE_LEN),
+        ("frontier", c_uint32),
+        ("reserved", c_uint32),
     ]
 
 

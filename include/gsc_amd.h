@@ -295,11 +295,20 @@ typedef struct gsc_solve_summary {
     double reduction_ms;         /* sum of the per-iteration bookkeeping kernel time */
     uint64_t pairs_tested;       /* pairs whose record was read and tested, all iterations */
     uint64_t pair_bytes_read;    /* algorithmic bytes of pair records streamed, all iterations */
-    uint64_t boxes_counted;      /* tests that needed the full 16-corner prefix count */
+    uint64_t boxes_counted;      /* pair tests that needed a 16-corner prefix count */
     uint64_t test_slots;         /* lane slots spent on tests (pairs_tested / test_slots =
                                     lane utilisation of the satisfaction pass) */
     char message[GSC_MESSAGE_LEN];
+    /* How unchanged states were skipped: GSC_FRONTIER_NONE (every unsettled
+       state tested every iteration: the sweep schedule, or wrap on x or y),
+       GSC_FRONTIER_PREFIX (hull counts on prefix tables) or GSC_FRONTIER_BITS
+       (per-plane bitmasks; every box settled by bitmask row scans). */
+    uint32_t frontier;
+    uint32_t reserved;
 } gsc_solve_summary;
+#define GSC_FRONTIER_NONE 0
+#define GSC_FRONTIER_PREFIX 1
+#define GSC_FRONTIER_BITS 2
 
 void gsc_solver_default_options(gsc_solver_options* opts);
 int gsc_solver_create(const gsc_case_desc* cfg, const gsc_solver_options* opts,

@@ -167,3 +167,33 @@ def test_overflow_table_regrowth(gpu, oracle):
     wrapped = va & ((mx % n0) < (mn % n0))
     assert wrapped.sum() > 65536  # the case really needs the regrowth
     assert got.memory_stats["abstraction_bytes"] >= case.state_count() * case.input_count() * 4 + wrapped.sum() * 16
+
+
+# Frontier skipping has three modes (gsc_solve_summary::frontier): per-plane
+# bitmasks where every box fits the bitmask row scans, prefix-table hull
+# counts otherwise, and none under wrap on x or y. Each mode is pinned on
+# cases that take it, and each result is checked against the oracle, so that
+# a change that moves a case to another mode shows here as well as in timing.
+_FRONTIER_CASES = [
+    ("builtin:default", 2), ("builtin:wraparound", 2), ("c1", 2), ("small:2", 2), ("small:9", 2),
+    ("small:3", 1), ("small:19", 1), ("small:5", 0), ("small:21", 0),
+]
+
+
+@pytest.mark.parametrize("which,mode", _FRONTIER_CASES, ids=[w for w, _ in _FRONTIER_CASES])
+def test_frontier_mode(gpu, oracle, which, mode):
+    kind, _, arg = which.partition(":")
+    if kind == "builtin":
+        case = cases.builtin(arg)
+    elif kind == "c1":
+        case = cases.unicycle_30x30x24x10(sample_time=0.5)
+    else:
+        case = small_random_case(int(arg))
+    print(f"[case] {which} wrap_dim={case.wrap_dim}", flush=True)
+    ref = oracle.run(case)
+    with Solver(case, time_kernels=False) as s:
+        s.build_abstraction()
+        summary = s.solve()
+        got = s.download(abstraction=False)
+    assert summary["frontier"] == mode, f"{which}: frontier mode {summary['frontier']}, expected {mode}"
+    assert_same_solution(got, ref, which)
