@@ -462,7 +462,7 @@ void gsc_batch_default_options(gsc_batch_options* o) {
     if (!o) return;
     std::memset(o, 0, sizeof(*o));
     o->device = 0;
-    o->tile_wg = 256;
+    o->tile_wg = 512;
     o->tile_ib = 8;
     o->iterations_per_launch = 8;
     o->order = 1;
@@ -548,7 +548,7 @@ int gsc_batch_tune(gsc_batch* b, gsc_batch_tune_entry* entries, uint32_t capacit
         gsc_batch_options best = b->opts;
         best.use_graph = 1;
         double best_ms = 1e300;
-        for (int wg : {128, 256, 512})
+        for (int wg : {128, 256, 512, 1024})
             for (int ib : {4, 8}) {
                 gsc_batch_options o = best;
                 o.tile_wg = wg;

@@ -1473,11 +1473,10 @@ __device__ __forceinline__ bool chunk_changed(const SatisfyArgs& A, uint4 ch, ui
 // any state.
 // kSweep: the sweep_all_pairs schedule, where each lane counts its own pairs
 // in input order instead of through the count queue.
-template <uint32_t WG, uint32_t IB, bool kSweep, bool kBits>
+template <uint32_t WG, uint32_t kSpan, uint32_t IB, bool kSweep, bool kBits>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) void satisfy_kernel(SatisfyArgs A,
                                                                                               uint32_t iter) {
     const uint32_t block = blockIdx.x;
-    constexpr bool kChunkFlags = true;
 #include "satisfy_body.inc"
 }
 
@@ -2099,21 +2098,20 @@ __global__ __launch_bounds__(kFrontierThreads) void frontier_plane_batch_kernel(
 // function boundary gives the better register allocation (74 vs 97 VGPRs).
 // kBits: the batch runs the bitmask frontier (every case qualifies), else the
 // prefix-table stages and chunk flags.
-template <uint32_t WG, uint32_t IB, bool kBits>
+template <uint32_t WG, uint32_t kSpan, uint32_t IB, bool kBits>
 __device__ __forceinline__ void satisfy_in_batch(const SatisfyArgs& A, uint32_t block, uint32_t iter) {
-    constexpr bool kChunkFlags = true;  // this iteration's chunk flags or frontier words are written
-    constexpr bool kSweep = false;      // the count queue in both schedules
+    constexpr bool kSweep = false;  // the count queue in both schedules
 #include "satisfy_body.inc"
 }
 
-template <uint32_t WG, uint32_t IB, bool kBits>
+template <uint32_t WG, uint32_t kSpan, uint32_t IB, bool kBits>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7))) void satisfy_batch_kernel(const BatchInstance* __restrict__ B,
                                                            const uint32_t* __restrict__ starts, uint32_t n) {
     const uint32_t inst = find_instance(starts, n, blockIdx.x);
     const SatisfyArgs& A = B[inst].sat;
     const uint64_t st = loop_state(A.ctl);
     if (state_done(st)) return;
-    satisfy_in_batch<WG, IB, kBits>(A, blockIdx.x - starts[inst], state_next_iteration(st));
+    satisfy_in_batch<WG, kSpan, IB, kBits>(A, blockIdx.x - starts[inst], state_next_iteration(st));
 }
 
 __global__ void finish_batch_kernel(const BatchInstance* __restrict__ B, uint32_t n, BatchStatus* status) {
@@ -2156,33 +2154,36 @@ namespace {
 using SatSingle = void (*)(SatisfyArgs, uint32_t);
 using SatBatch = void (*)(const BatchInstance*, const uint32_t*, uint32_t);
 
-template <uint32_t WG>
+template <uint32_t WG, uint32_t kSpan>
 SatSingle pick_single(uint32_t ib, bool sweep, bool bits) {
-    if (sweep) return ib == 4 ? satisfy_kernel<WG, 4, true, false> : satisfy_kernel<WG, 8, true, false>;
-    if (bits) return ib == 4 ? satisfy_kernel<WG, 4, false, true> : satisfy_kernel<WG, 8, false, true>;
-    return ib == 4 ? satisfy_kernel<WG, 4, false, false> : satisfy_kernel<WG, 8, false, false>;
+    if (sweep) return ib == 4 ? satisfy_kernel<WG, kSpan, 4, true, false> : satisfy_kernel<WG, kSpan, 8, true, false>;
+    if (bits) return ib == 4 ? satisfy_kernel<WG, kSpan, 4, false, true> : satisfy_kernel<WG, kSpan, 8, false, true>;
+    return ib == 4 ? satisfy_kernel<WG, kSpan, 4, false, false> : satisfy_kernel<WG, kSpan, 8, false, false>;
 }
-template <uint32_t WG>
+template <uint32_t WG, uint32_t kSpan>
 SatBatch pick_batch(uint32_t ib, bool bits) {
-    if (bits) return ib == 4 ? satisfy_batch_kernel<WG, 4, true> : satisfy_batch_kernel<WG, 8, true>;
-    return ib == 4 ? satisfy_batch_kernel<WG, 4, false> : satisfy_batch_kernel<WG, 8, false>;
+    if (bits) return ib == 4 ? satisfy_batch_kernel<WG, kSpan, 4, true> : satisfy_batch_kernel<WG, kSpan, 8, true>;
+    return ib == 4 ? satisfy_batch_kernel<WG, kSpan, 4, false> : satisfy_batch_kernel<WG, kSpan, 8, false>;
 }
 }  // namespace
 
-bool satisfy_variant_supported(uint32_t wg, uint32_t ib) {
-    return (wg == 128 || wg == 256 || wg == 512) && (ib == 4 || ib == 8);
+bool satisfy_variant_supported(uint32_t cells, uint32_t ib) {
+    return (cells == 128 || cells == 256 || cells == 512 || cells == 1024) && (ib == 4 || ib == 8);
 }
+
+uint32_t satisfy_threads(uint32_t cells) { return cells < 256u ? cells : 256u; }
 
 uint32_t satisfy_blocks(const DevCase& c, uint32_t wg) {
     return (c.PQ + wg - 1) / wg * c.shape[2] * c.shape[3];
 }
 
-hipError_t launch_satisfy(const SatisfyArgs& A, uint32_t iter, uint32_t wg, uint32_t ib, hipStream_t stream) {
+hipError_t launch_satisfy(const SatisfyArgs& A, uint32_t iter, uint32_t cells, uint32_t ib, hipStream_t stream) {
     const bool sw = A.sweep_all != 0, bits = A.frontier_bits != nullptr;
-    const SatSingle f = wg == 128   ? pick_single<128>(ib, sw, bits)
-                        : wg == 512 ? pick_single<512>(ib, sw, bits)
-                                    : pick_single<256>(ib, sw, bits);
-    hipLaunchKernelGGL(f, dim3(satisfy_blocks(A.c, wg)), dim3(wg), 0, stream, A, iter);
+    const SatSingle f = cells == 128    ? pick_single<128, 1>(ib, sw, bits)
+                        : cells == 512  ? pick_single<256, 2>(ib, sw, bits)
+                        : cells == 1024 ? pick_single<256, 4>(ib, sw, bits)
+                                        : pick_single<256, 1>(ib, sw, bits);
+    hipLaunchKernelGGL(f, dim3(satisfy_blocks(A.c, cells)), dim3(satisfy_threads(cells)), 0, stream, A, iter);
     return hipGetLastError();
 }
 
@@ -2199,11 +2200,12 @@ hipError_t launch_chunk_flags(const SatisfyArgs& A, uint8_t* flags, uint32_t ite
 }
 
 hipError_t launch_satisfy_batch(const BatchInstance* B, const uint32_t* starts, uint32_t n, uint32_t blocks,
-                                uint32_t wg, uint32_t ib, bool bits, hipStream_t stream) {
-    const SatBatch f = wg == 128   ? pick_batch<128>(ib, bits)
-                       : wg == 512 ? pick_batch<512>(ib, bits)
-                                   : pick_batch<256>(ib, bits);
-    hipLaunchKernelGGL(f, dim3(blocks), dim3(wg), 0, stream, B, starts, n);
+                                uint32_t cells, uint32_t ib, bool bits, hipStream_t stream) {
+    const SatBatch f = cells == 128    ? pick_batch<128, 1>(ib, bits)
+                       : cells == 512  ? pick_batch<256, 2>(ib, bits)
+                       : cells == 1024 ? pick_batch<256, 4>(ib, bits)
+                                       : pick_batch<256, 1>(ib, bits);
+    hipLaunchKernelGGL(f, dim3(blocks), dim3(satisfy_threads(cells)), 0, stream, B, starts, n);
     return hipGetLastError();
 }
 

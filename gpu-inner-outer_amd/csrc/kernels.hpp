@@ -253,9 +253,11 @@ __global__ void finish_kernel(IterCtl* ctl, IterRecord* records, uint32_t max_it
 
 // Satisfaction tile variants: WG cells (and threads) per workgroup, IB inputs per
 // block of record loads.
-bool satisfy_variant_supported(uint32_t wg, uint32_t ib);
-uint32_t satisfy_blocks(const DevCase& c, uint32_t wg);
-hipError_t launch_satisfy(const SatisfyArgs& A, uint32_t iter, uint32_t wg, uint32_t ib, hipStream_t stream);
+bool satisfy_variant_supported(uint32_t cells, uint32_t ib);
+// Satisfaction tile: `cells` per workgroup (128, 256, 512 or 1024; 256 threads
+// with 2 or 4 64-cell chunks per wave above 256) and `ib` inputs per block.
+uint32_t satisfy_blocks(const DevCase& c, uint32_t cells);
+hipError_t launch_satisfy(const SatisfyArgs& A, uint32_t iter, uint32_t cells, uint32_t ib, hipStream_t stream);
 // Chunk flags: row pitch per plane (a multiple of 16, so that a workgroup of up
 // to 16 waves reads its flags as one aligned word) and the launch that fills
 // them for iteration `iter` from the chunk hulls and the coarse tables.

@@ -668,7 +668,7 @@ void gsc_server_default_options(gsc_server_options* o) {
     o->device = 0;
     o->max_active = 64;
     o->iterations_per_step = 4;
-    o->tile_wg = 256;
+    o->tile_wg = 512;
     o->tile_ib = 8;
     o->result_pool_mib = 1024;
     o->hbm_budget_bytes = 64ull << 30;

@@ -959,6 +959,14 @@ int gsc_solver_create(const gsc_case_desc* cfg, const gsc_solver_options* opts, 
         s->opts = *opts;
     else
         gsc_solver_default_options(&s->opts);
+    const int32_t wg = s->opts.tile_wg ? s->opts.tile_wg : 512, ib = s->opts.tile_ib ? s->opts.tile_ib : 8;
+    if (wg < 0 || ib < 0 || !satisfy_variant_supported(static_cast<uint32_t>(wg), static_cast<uint32_t>(ib))) {
+        put(err, err_len, "tile_wg must be 128, 256, 512 or 1024 and tile_ib 4 or 8");
+        delete s;
+        return GSC_ERR_INVALID_ARGUMENT;
+    }
+    s->tile_wg = static_cast<uint32_t>(wg);
+    s->tile_ib = static_cast<uint32_t>(ib);
     copy_case(s, *cfg);
     try {
         s->setup();

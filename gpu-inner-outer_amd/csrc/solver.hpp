@@ -305,7 +305,7 @@ struct gsc_solver {
 
     uint64_t candidate_count = 0;
     // Satisfaction tile: cells per workgroup and inputs per block of record loads.
-    uint32_t tile_wg = 256, tile_ib = 8;
+    uint32_t tile_wg = 512, tile_ib = 8;
     uint32_t plane_pitch = 0;  // LDS row pitch of the fused prefix build; 0: separate passes
     bool abstraction_ready = false;
     bool solved = false;

@@ -199,7 +199,8 @@ class SolverOptions(ctypes.Structure):
         ("sweep_all_pairs", c_int32),
         ("verbose", c_int32),
         ("recompute_abstraction", c_int32),
-        ("reserved", c_int32 * 2),
+        ("tile_wg", c_int32),
+        ("tile_ib", c_int32),
     ]
 
 

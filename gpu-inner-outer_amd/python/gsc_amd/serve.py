@@ -25,7 +25,7 @@ from .solver import GscError, RunResult, _make_buffers, _result_from, _struct_di
 
 class Server:
     def __init__(self, device: int = 0, max_active: int = 64, iterations_per_step: int = 4,
-                 hbm_budget_bytes: int = 64 << 30, tile_wg: int = 256, tile_ib: int = 8):
+                 hbm_budget_bytes: int = 64 << 30, tile_wg: int = 512, tile_ib: int = 8):
         self.lib = capi.load()
         o = capi.ServerOptions()
         self.lib.gsc_server_default_options(ctypes.byref(o))

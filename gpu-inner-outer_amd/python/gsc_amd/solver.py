@@ -164,7 +164,8 @@ class Solver:
     """One case resident on one device (gsc_solver_* entry points)."""
 
     def __init__(self, case: Case, device: int = 0, keep_export: bool = False, time_kernels: bool = True,
-                 sweep_all_pairs: bool = False, verbose: bool = False, recompute_abstraction: bool = False):
+                 sweep_all_pairs: bool = False, verbose: bool = False, recompute_abstraction: bool = False,
+                 tile_wg: int = 0, tile_ib: int = 0):
         self.lib = capi.load()
         self.case = case
         opts = capi.SolverOptions()
@@ -175,6 +176,7 @@ class Solver:
         opts.sweep_all_pairs = int(sweep_all_pairs)
         opts.verbose = int(verbose)
         opts.recompute_abstraction = int(recompute_abstraction)
+        opts.tile_wg, opts.tile_ib = int(tile_wg), int(tile_ib)
         d, keep = case.to_desc()
         h = ctypes.c_void_p()
         err = ctypes.create_string_buffer(512)

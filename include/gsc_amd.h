@@ -279,7 +279,10 @@ typedef struct gsc_solver_options {
     int32_t verbose;         /* print per-iteration progress like kernels.cu:746-756 */
     int32_t recompute_abstraction; /* 1: store no pair records; every iteration recomputes the
                                       successor boxes it tests in FP64 (same results) */
-    int32_t reserved[2];
+    int32_t tile_wg;         /* cells per satisfaction workgroup: 128, 256, 512 or 1024
+                                (at most 256 threads; 64-cell chunks per wave above 256);
+                                0: 512 */
+    int32_t tile_ib;         /* inputs per block of record loads: 4 or 8; 0: 8 */
 } gsc_solver_options;
 
 typedef struct gsc_solve_summary {
@@ -353,7 +356,7 @@ typedef struct gsc_batch gsc_batch;
 
 typedef struct gsc_batch_options {
     int32_t device;
-    int32_t tile_wg;               /* cells (and threads) per satisfaction workgroup: 128, 256, 512 */
+    int32_t tile_wg;               /* cells per satisfaction workgroup: 128, 256, 512, 1024 */
     int32_t tile_ib;               /* inputs per block of record loads: 4 or 8 */
     int32_t iterations_per_launch; /* iterations per graph replay, 1..64 */
     int32_t order;                 /* 0: cases in the given order, 1: largest pair count first */

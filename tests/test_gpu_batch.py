@@ -45,7 +45,7 @@ def test_batch_matches_oracle(batch_and_refs):
     _check_all(b, cs, refs, "default")
 
 
-@pytest.mark.parametrize("wg", [128, 256, 512])
+@pytest.mark.parametrize("wg", [128, 256, 512, 1024])
 @pytest.mark.parametrize("ib", [4, 8])
 def test_batch_tile_variants(batch_and_refs, wg, ib):
     b, cs, refs = batch_and_refs

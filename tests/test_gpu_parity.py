@@ -197,3 +197,38 @@ def test_frontier_mode(gpu, oracle, which, mode):
         got = s.download(abstraction=False)
     assert summary["frontier"] == mode, f"{which}: frontier mode {summary['frontier']}, expected {mode}"
     assert_same_solution(got, ref, which)
+
+
+# Every satisfaction tile (cells per workgroup, inputs per block) on cases that
+# run each frontier mode: the bitmask frontier (builtin default, c1, small:2),
+# the prefix-table frontier (small:3) and none (small:5, x/y wrap), and the
+# sweep schedule on one of them.
+_TILE_CASES = ["builtin:default", "c1", "small:2", "small:3", "small:5"]
+
+
+@pytest.mark.parametrize("cells", [128, 256, 512, 1024])
+@pytest.mark.parametrize("ib", [4, 8])
+def test_satisfy_tiles(gpu, oracle, cells, ib):
+    for which in _TILE_CASES:
+        kind, _, arg = which.partition(":")
+        if kind == "builtin":
+            case = cases.builtin(arg)
+        elif kind == "c1":
+            case = cases.unicycle_30x30x24x10(sample_time=0.5)
+        else:
+            case = small_random_case(int(arg))
+        print(f"[case] {which} tile {cells}x{ib}", flush=True)
+        ref = oracle.run(case)
+        for sweep in ((False, True) if which == "small:2" else (False,)):
+            with Solver(case, time_kernels=False, sweep_all_pairs=sweep, tile_wg=cells, tile_ib=ib) as s:
+                s.build_abstraction()
+                s.solve()
+                got = s.download(abstraction=False)
+            assert_same_solution(got, ref, f"{which} tile {cells}x{ib} sweep={sweep}")
+
+
+def test_satisfy_tile_rejected(gpu):
+    with pytest.raises(GscError):
+        Solver(cases.builtin("default"), tile_wg=64)
+    with pytest.raises(GscError):
+        Solver(cases.builtin("default"), tile_ib=16)
