@@ -288,7 +288,7 @@ def batch_direct_section(count=32, seed=3, steps=2):
     from gsc_amd.batch import Batch
 
     cs = cases.batch_instances(count, seed=seed)
-    with Batch(cs, tile_wg=256, tile_ib=8, iterations_per_launch=32, order=1, use_graph=0) as b:
+    with Batch(cs, tile_wg=1024, tile_ib=8, iterations_per_launch=32, order=1, use_graph=0) as b:
         b.build_abstraction()
         b.solve()
         ms = []

@@ -959,7 +959,11 @@ int gsc_solver_create(const gsc_case_desc* cfg, const gsc_solver_options* opts, 
         s->opts = *opts;
     else
         gsc_solver_default_options(&s->opts);
-    const int32_t wg = s->opts.tile_wg ? s->opts.tile_wg : 512, ib = s->opts.tile_ib ? s->opts.tile_ib : 8;
+    // Default tile: 512 cells, where the tested states are sparse enough that a
+    // group's four waves share a few batches; 256 under sweep_all_pairs, where
+    // every state is tested and larger groups would run their batches in turn.
+    const int32_t wg = s->opts.tile_wg ? s->opts.tile_wg : s->opts.sweep_all_pairs ? 256 : 512;
+    const int32_t ib = s->opts.tile_ib ? s->opts.tile_ib : 8;
     if (wg < 0 || ib < 0 || !satisfy_variant_supported(static_cast<uint32_t>(wg), static_cast<uint32_t>(ib))) {
         put(err, err_len, "tile_wg must be 128, 256, 512 or 1024 and tile_ib 4 or 8");
         delete s;

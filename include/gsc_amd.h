@@ -281,7 +281,7 @@ typedef struct gsc_solver_options {
                                       successor boxes it tests in FP64 (same results) */
     int32_t tile_wg;         /* cells per satisfaction workgroup: 128, 256, 512 or 1024
                                 (at most 256 threads; 64-cell chunks per wave above 256);
-                                0: 512 */
+                                0: 512, or 256 with sweep_all_pairs */
     int32_t tile_ib;         /* inputs per block of record loads: 4 or 8; 0: 8 */
 } gsc_solver_options;
 
