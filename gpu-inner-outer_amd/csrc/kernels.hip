@@ -1264,8 +1264,10 @@ __device__ __forceinline__ bool rows_any_16(const uint32_t* __restrict__ bm, uin
 // 16-corner counts (count_test). With kBits (frontier skipping on bitmasks,
 // where no unreached prefix table is built) the solver guarantees that every
 // box is compact and fits the scan. `counted` counts the pairs that reach a
-// prefix-table count.
-template <bool kBits>
+// prefix-table count. kScan false (the sweep schedule, where most tested
+// pairs are unsatisfied and the first and last rows usually show it) skips
+// the whole-box scan in favour of those two rows and the count.
+template <bool kBits, bool kScan = true>
 __device__ __forceinline__ bool pair_satisfied(const SatisfyArgs& A, const uint32_t* PU, uint32_t* rec_plane,
                                                const uint32_t* pb_plane, uint32_t plane, uint32_t qs, uint32_t i,
                                                uint32_t& counted) {
@@ -1291,7 +1293,7 @@ __device__ __forceinline__ bool pair_satisfied(const SatisfyArgs& A, const uint3
         const uint32_t nz = range_cells(b2, n2), nw = range_cells(b3, n3);
         const bool orb = pbi >= probe_or_bit0(c);
         const uint32_t sz = orb ? nz - 1u : nz, sw = orb ? nw - 1u : nw;
-        if (kBits || (e1 < 16u && sz <= 2u && sw <= 2u)) {
+        if (kBits || (kScan && e1 < 16u && sz <= 2u && sw <= 2u)) {
             const uint32_t pitch = orb ? probe_or_pitch(c) : c.PQ;
             // The other slices start one θ cell up (sz = 2), one v cell up
             // (sw = 2), or both.
