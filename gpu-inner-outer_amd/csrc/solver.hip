@@ -454,17 +454,45 @@ void gsc_solver::upload_tables() {
             set_cell_filter(dc, d, dc.K, x_abs, abs_sum, rmax[d]);
         }
     }
-    if (!t.cs.empty())
-        GSC_HIP(hipMemcpyAsync(cs, t.cs.data(), t.cs.size() * sizeof(double2), hipMemcpyHostToDevice, stream));
-    if (!t.vt.empty())
-        GSC_HIP(hipMemcpyAsync(vt, t.vt.data(), t.vt.size() * sizeof(double), hipMemcpyHostToDevice, stream));
-    GSC_HIP(hipMemcpyAsync(box2, t.box2.data(), t.box2.size() * sizeof(uint2), hipMemcpyHostToDevice, stream));
-    GSC_HIP(hipMemcpyAsync(box3, t.box3.data(), t.box3.size() * sizeof(uint2), hipMemcpyHostToDevice, stream));
-    GSC_HIP(hipMemcpyAsync(hull2, t.hull2.data(), t.hull2.size() * sizeof(uint2), hipMemcpyHostToDevice, stream));
-    GSC_HIP(hipMemcpyAsync(hull3, t.hull3.data(), t.hull3.size() * sizeof(uint2), hipMemcpyHostToDevice, stream));
-    GSC_HIP(hipMemcpyAsync(rad, t.rad.data(), t.rad.size() * sizeof(double), hipMemcpyHostToDevice, stream));
-    if (fail2d)
-        GSC_HIP(hipMemcpyAsync(fail2d, t.fail.data(), t.fail.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    // The uploads. From pageable memory every copy is staged by the runtime
+    // while the host waits, so the tables are packed into one pinned buffer
+    // first (kept for later abstractions; the previous one's copies have
+    // ended, since finish_abstraction synchronises the stream). Without
+    // allocation they go straight from the host vectors.
+    struct Upload {
+        void* dst;
+        const void* src;
+        size_t bytes;
+    };
+    const Upload ups[] = {{cs, t.cs.data(), t.cs.size() * sizeof(double2)},
+                          {vt, t.vt.data(), t.vt.size() * sizeof(double)},
+                          {box2, t.box2.data(), t.box2.size() * sizeof(uint2)},
+                          {box3, t.box3.data(), t.box3.size() * sizeof(uint2)},
+                          {hull2, t.hull2.data(), t.hull2.size() * sizeof(uint2)},
+                          {hull3, t.hull3.data(), t.hull3.size() * sizeof(uint2)},
+                          {rad, t.rad.data(), t.rad.size() * sizeof(double)},
+                          {fail2d, t.fail.data(), fail2d ? t.fail.size() * sizeof(uint32_t) : 0}};
+    const auto aligned = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
+    size_t total = 0;
+    for (const Upload& u : ups) total += aligned(u.bytes);
+    if (!no_alloc && table_stage_bytes < total) {
+        if (table_stage) GSC_HIP(hipHostFree(table_stage));
+        table_stage = nullptr;
+        table_stage_bytes = 0;
+        GSC_HIP(hipHostMalloc(reinterpret_cast<void**>(&table_stage), total, hipHostMallocDefault));
+        table_stage_bytes = total;
+    }
+    size_t off = 0;
+    for (const Upload& u : ups) {
+        if (u.bytes == 0 || u.dst == nullptr) continue;
+        const void* src = u.src;
+        if (table_stage) {
+            std::memcpy(table_stage + off, u.src, u.bytes);
+            src = table_stage + off;
+            off += aligned(u.bytes);
+        }
+        GSC_HIP(hipMemcpyAsync(u.dst, src, u.bytes, hipMemcpyHostToDevice, stream));
+    }
     const uint64_t n_planes = static_cast<uint64_t>(dc.shape[2]) * dc.shape[3];
     const uint64_t n_inc = n_planes * dc.I * dc.K;
     if (n_inc) {

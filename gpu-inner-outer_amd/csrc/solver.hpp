@@ -275,6 +275,10 @@ struct gsc_solver {
     IterCtl* host_ctl = nullptr;        // pinned, allocated by the first solve()
     IterRecord* host_records = nullptr; // pinned, allocated by the first solve()
     uint32_t* host_words = nullptr;     // pinned scratch words (overflow count, mask counts)
+    // Pinned staging of the model tables (one buffer, each table 256-B aligned),
+    // allocated by the first abstraction and kept; null without allocation.
+    unsigned char* table_stage = nullptr;
+    size_t table_stage_bytes = 0;
     bool owns_host_words = true;        // false when a server lends them from its pool
     bool owns_host_ctl = true;          // false when host_ctl / host_records are lent
     std::vector<hipEvent_t> events;
@@ -362,6 +366,7 @@ struct gsc_solver {
             if (st.hval) (void)hipHostFree(st.hval);
             if (st.done) (void)hipEventDestroy(st.done);
         }
+        if (table_stage) (void)hipHostFree(table_stage);
         if (host_ctl && owns_host_ctl) (void)hipHostFree(host_ctl);
         if (host_records && owns_host_ctl) (void)hipHostFree(host_records);
         if (host_words && owns_host_words) (void)hipHostFree(host_words);
