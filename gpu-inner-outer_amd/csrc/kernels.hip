@@ -108,16 +108,19 @@ __device__ inline void unflatten(const DevCase& c, uint32_t s, uint32_t idx[4]) 
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// Masks and seeds. Thread per state.
+// Masks and seeds. Grid-stride over the states (coalesced byte stores per
+// pass); each thread counts its candidates and bad candidates, and each
+// workgroup adds its sums with one atomic per counter (per-wave atomics on the
+// two counters serialised the kernel: 0.44 ms at 4M states).
 // `allow` (may be null) removes states a caller-side pointwise constraint rejects.
 __global__ __launch_bounds__(256) void masks_kernel(DevCase c, const gsc_rect4* __restrict__ obstacles,
                                                      const gsc_rect4* __restrict__ targets,
                                                      const uint8_t* __restrict__ allow,
                                                      uint8_t* __restrict__ valid, uint8_t* __restrict__ cand,
                                                      unsigned long long* __restrict__ counts) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    bool is_cand = false, bad = false;
-    if (s < c.S) {
+    uint64_t n_cand = 0, n_bad = 0;
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < c.S; s += gridDim.x * blockDim.x) {
+        bool is_cand = false, bad = false;
         uint32_t idx[4];
         unflatten(c, s, idx);
         double x[4];
@@ -143,12 +146,26 @@ __global__ __launch_bounds__(256) void masks_kernel(DevCase c, const gsc_rect4* 
         valid[s] = ok ? 1 : 0;
         cand[s] = is_cand ? 1 : 0;
         bad = is_cand && !ok;
+        n_cand += is_cand ? 1u : 0u;
+        n_bad += bad ? 1u : 0u;
     }
-    const uint64_t nc = __popcll(__ballot(is_cand));
-    const uint64_t nb = __popcll(__ballot(bad));
+    __shared__ unsigned long long s_sum[2][4];
+    n_cand = wave_sum_u64(n_cand);
+    n_bad = wave_sum_u64(n_bad);
+    const uint32_t wave = threadIdx.x >> 6;
     if (lane_id() == 0) {
-        if (nc) atomicAdd(&counts[0], static_cast<unsigned long long>(nc));
-        if (nb) atomicAdd(&counts[1], static_cast<unsigned long long>(nb));
+        s_sum[0][wave] = n_cand;
+        s_sum[1][wave] = n_bad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long a = 0, b = 0;
+        for (uint32_t w = 0; w < blockDim.x / 64u; ++w) {
+            a += s_sum[0][w];
+            b += s_sum[1][w];
+        }
+        if (a) atomicAdd(&counts[0], a);
+        if (b) atomicAdd(&counts[1], b);
     }
 }
 

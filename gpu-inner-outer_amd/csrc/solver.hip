@@ -317,7 +317,7 @@ void gsc_solver::setup_launch() {
 
     // Masks on the device (the reference builds them in a serial host loop even in
     // GPU mode, abstraction.hpp:257-265).
-    hipLaunchKernelGGL(masks_kernel, dim3(blocks_for(S, 256)), dim3(256), 0, stream, dc, d_obstacles, d_targets, d_allow, valid,
+    hipLaunchKernelGGL(masks_kernel, dim3(std::min<uint64_t>(blocks_for(S, 256), 2048)), dim3(256), 0, stream, dc, d_obstacles, d_targets, d_allow, valid,
                        cand, mask_counts);
     GSC_HIP(hipGetLastError());
     unsigned long long* counts = reinterpret_cast<unsigned long long*>(host_words + kWordMaskCounts);
