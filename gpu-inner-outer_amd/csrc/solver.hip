@@ -458,7 +458,9 @@ void gsc_solver::upload_tables() {
     // while the host waits, so the tables are packed into one pinned buffer
     // first (kept for later abstractions; the previous one's copies have
     // ended, since finish_abstraction synchronises the stream). Without
-    // allocation they go straight from the host vectors.
+    // allocation, and for a solver that draws on a server's arena (whose
+    // admission path must not allocate host memory either), they go straight
+    // from the host vectors.
     struct Upload {
         void* dst;
         const void* src;
@@ -475,7 +477,7 @@ void gsc_solver::upload_tables() {
     const auto aligned = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
     size_t total = 0;
     for (const Upload& u : ups) total += aligned(u.bytes);
-    if (!no_alloc && table_stage_bytes < total) {
+    if (!no_alloc && arena == nullptr && table_stage_bytes < total) {
         if (table_stage) GSC_HIP(hipHostFree(table_stage));
         table_stage = nullptr;
         table_stage_bytes = 0;
