@@ -170,7 +170,7 @@ __host__ __device__ inline uint32_t frontier_lds_bytes(const DevCase& c) {
     return 2u * c.shape[1] * ((c.shape[0] + 31u) / 32u) * 4u;
 }
 constexpr uint32_t kFrontierLdsMax = 65536;
-constexpr uint32_t kFrontierThreads = 256;
+constexpr uint32_t kFrontierThreads = 512;
 
 
 // One instance of a multi-instance (batched) iteration: its satisfaction
